@@ -1,0 +1,150 @@
+"""Python mirror of the reference module `AIRMs` (AIRMs.f90).
+
+`Opts` is `type(opts)` (:6-57) with initialise / set_ubound / set_lbound; `sirt`,
+`gradient_descent`, `normest` and `art` keep the reference's argument meaning.  Where the
+reference draws from the compiler's RANDOM_NUMBER (normest's start vector, randomized
+Kaczmarz), the caller supplies the draws: `v0` for normest, `uniforms` for art, in the order
+the reference consumes them (`art_uniform_count` tells how many).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FairError, FAIR_ERR_ARG, FAIR_ERR_METHOD, check, lib
+from .sparse_matrices import CSRMatrix, _f64, _ptr
+
+SIRT_METHODS = {"cimmino": 0, "sart": 1, "sirt": 1}
+ART_METHODS = {"kaczmarz": 0, "symkaczmarz": 1, "sym": 1, "randkaczmarz": 2, "rand": 2}
+
+
+class Opts:
+    """type(opts), AIRMs.f90:6-57."""
+
+    def __init__(self):
+        self.initialise()
+
+    def initialise(self):
+        """initialise_options (AIRMs.f90:17-37): note the placeholder ubound=0, lbound=1."""
+        self.upb = False
+        self.lwrb = False
+        self.ubound = 0.0
+        self.lbound = 1.0
+        self.damp = 0.0
+        self.relaxpar = 1.0
+        self.verbose = False
+        self.rowprobability = False
+        return self
+
+    def set_ubound(self, ub: float):
+        self.upb = True
+        self.ubound = float(ub)
+        return self
+
+    def set_lbound(self, lb: float):
+        self.lwrb = True
+        self.lbound = float(lb)
+        return self
+
+    def _c(self) -> _lib.Opts:
+        return _lib.Opts(float(self.relaxpar), float(self.ubound), float(self.lbound), float(self.damp),
+                         int(bool(self.verbose)), int(bool(self.upb)), int(bool(self.lwrb)),
+                         int(bool(self.rowprobability)))
+
+
+def _opts(options):
+    return (options or Opts())._c()
+
+
+def _maxval(Kin) -> int:
+    return int(np.max(np.atleast_1d(Kin)))
+
+
+def sirt(sirt_method: str, A: CSRMatrix, b, Kin, x0=None, options: Opts | None = None) -> np.ndarray:
+    """call sirt(xout, sirt_method, A, b, Kin, x0, options), AIRMs.f90:997-1127.  Runs
+    maxval(Kin) iterations; SART leaves A's rows sorted by column, as the reference does."""
+    if sirt_method not in SIRT_METHODS:
+        raise FairError(FAIR_ERR_METHOD, "Not a recognised SIRT method.")
+    b = _f64(b, A.m)
+    x0a = None if x0 is None else _f64(x0, A.n)
+    if x0 is not None and x0a.shape[0] != A.n:
+        raise FairError(FAIR_ERR_ARG, "Size of x0 should be equal to n (#columns).")
+    out = np.empty(A.n)
+    o = _opts(options)
+    check(lib().fair_sirt(A.handle, SIRT_METHODS[sirt_method], _ptr(b), _maxval(Kin), _ptr(x0a),
+                          C.byref(o), _ptr(out)))
+    return out
+
+
+def sirt_dev(sirt_method: str, A: CSRMatrix, d_b: int, maxits: int, d_x: int,
+             options: Opts | None = None) -> None:
+    """Device-resident SIRT on HBM vectors (d_x: start in, result out)."""
+    o = _opts(options)
+    check(lib().fair_sirt_dev(A.handle, SIRT_METHODS[sirt_method], C.c_void_p(d_b), int(maxits),
+                              C.c_void_p(d_x), C.byref(o)))
+
+
+def normest(A: CSRMatrix, tolerance: float = 1e-5, max_its: int = 100, v0=None, seed: int = 0) -> float:
+    """normest (AIRMs.f90:1129-1173): power iteration on A^T A from v0 (uniform [0,1) draws,
+    seeded here when not supplied)."""
+    v0 = np.random.default_rng(seed).random(A.n) if v0 is None else _f64(v0, A.n)
+    s = C.c_double()
+    check(lib().fair_normest(A.handle, _ptr(v0), float(tolerance), int(max_its), C.byref(s)))
+    return s.value
+
+
+def gradient_descent(A: CSRMatrix, b, max_iter: int = 100, x0=None, rho_in: float = 1.5,
+                     tolerance: float = 1e-6, options: Opts | None = None, sigma: float | None = None,
+                     v0=None, return_iters: bool = False):
+    """gradient_descent (AIRMs.f90:860-995).  sigma defaults to normest(A, v0=v0)."""
+    if tolerance < 0:
+        raise FairError(FAIR_ERR_ARG, "Must use positive tolerance.")
+    b = _f64(b, A.m)
+    x0a = None if x0 is None else _f64(x0, A.n)
+    if sigma is None:
+        sigma = normest(A, v0=v0)
+    out = np.empty(A.n)
+    its = C.c_int32()
+    o = _opts(options)
+    check(lib().fair_gradient_descent(A.handle, _ptr(b), int(max_iter), _ptr(x0a), float(rho_in),
+                                      float(tolerance), float(sigma), C.byref(o), _ptr(out), C.byref(its)))
+    return (out, its.value) if return_iters else out
+
+
+def _art_args(art_method, Kin, order):
+    if art_method not in ART_METHODS:
+        raise FairError(FAIR_ERR_METHOD, "Unknown ART method. Use kaczmarz, symkaczmarz or randkaczmarz")
+    K = np.ascontiguousarray(np.atleast_1d(Kin), np.int32)
+    ordv = None if order is None else np.ascontiguousarray(order, np.int32)
+    return ART_METHODS[art_method], K, ordv
+
+
+def art_uniform_count(art_method: str, A: CSRMatrix, Kin, order=None, options: Opts | None = None) -> int:
+    meth, K, ordv = _art_args(art_method, Kin, order)
+    o = _opts(options)
+    cnt = C.c_int64()
+    check(lib().fair_art_uniform_count(A.handle, meth, K.shape[0], _ptr(K), _ptr(ordv),
+                                       0 if ordv is None else ordv.shape[0], C.byref(o), C.byref(cnt)))
+    return cnt.value
+
+
+def art(art_method: str, A: CSRMatrix, b, Kin, x0=None, order=None, options: Opts | None = None,
+        uniforms=None, seed: int = 0) -> np.ndarray:
+    """call art(Xout, art_method, A, b, Kin, x0, order=order, options=options), AIRMs.f90:59-369
+    (without the xref early stop).  Returns Xout as an array of shape (size(Kin), n): row k is the
+    iterate recorded at Kin(k).  `uniforms` are the RANDOM_NUMBER draws of randomized Kaczmarz in
+    the reference's order; seeded uniform draws are used when not supplied."""
+    meth, K, ordv = _art_args(art_method, Kin, order)
+    b = _f64(b, A.m)
+    x0a = None if x0 is None else _f64(x0, A.n)
+    o = _opts(options)
+    if uniforms is None and meth == 2:
+        uniforms = np.random.default_rng(seed).random(art_uniform_count(art_method, A, Kin, order, options))
+    u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float64)
+    out = np.zeros((K.shape[0], A.n))
+    check(lib().fair_art(A.handle, meth, _ptr(b), K.shape[0], _ptr(K), _ptr(x0a), _ptr(ordv),
+                         0 if ordv is None else ordv.shape[0], C.byref(o), _ptr(u),
+                         0 if u is None else u.shape[0], _ptr(out)))
+    return out

@@ -1,0 +1,158 @@
+// fair_internal.hpp -- shared pieces of the gfx950 library: status handling, device buffers,
+// the device CSR object, wavefront reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/fair_hip.h"
+
+namespace fair {
+
+constexpr int WAVE = 64;  // CDNA wavefront width
+
+// ---- status ------------------------------------------------------------------------------
+void set_error(const std::string &msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define FAIR_HIP(expr)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw ::fair::Error{FAIR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define FAIR_CHECK(cond, code, msg)                      \
+    do {                                                 \
+        if (!(cond)) throw ::fair::Error{(code), (msg)}; \
+    } while (0)
+
+// Runs f() and converts exceptions into status codes.
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        return FAIR_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return FAIR_ERR_HIP;
+    }
+}
+
+hipStream_t stream();
+void launch_check(const char *what);
+
+// ---- device memory -------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) FAIR_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count) {
+        if (count) FAIR_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+    }
+    void download(T *h, size_t count) const {
+        if (count) FAIR_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// ---- device CSR ----------------------------------------------------------------------------
+// Row offsets IA (m+1, int32, zero-based), column indices JA (int32, ZERO-based on the device;
+// the host boundary is one-based), values vA (double).
+struct Csr {
+    int32_t m = 0, n = 0;
+    int64_t nnz = 0;
+    DBuf<int32_t> IA, JA;
+    DBuf<double> vA;
+    DBuf<int32_t> rows_nz;   // compacted list of non-empty rows, int4 {row, start, end, 0}
+    int32_t n_rows_nz = -1;  // -1: list not built
+    DBuf<int32_t> rows_empty;  // ids of the empty rows
+    int32_t n_rows_empty = 0;
+    int32_t max_row = 0;     // longest row
+};
+
+}  // namespace fair
+
+struct fair_csr {
+    fair::Csr A;        // the matrix as the caller sees it
+    fair::Csr T;        // CSR of A^T, built on demand
+    bool have_T = false;
+    bool rows_sorted = false;
+    // SIRT weights cached across calls (index: FAIR_CIMMINO / FAIR_SART) and loop workspace;
+    // dropped whenever the matrix changes.
+    fair::DBuf<double> M[2], D;
+    bool w_ok[2] = {false, false};
+    fair::DBuf<double> ws_r, ws_x;
+    void drop_cache() {
+        w_ok[0] = w_ok[1] = false;
+    }
+};
+
+namespace fair {
+
+// ---- kernels shared across translation units (defined in sparse.hip) -----------------------
+void build_row_list(Csr &A);
+void build_transpose(const Csr &A, Csr &T);      // T = CSR(A^T), rows sorted by column
+void sort_rows(Csr &A);                           // sort each row's entries by column
+void ensure_T(fair_csr *h);
+void invalidate_T(fair_csr *h);
+
+// y[r] = sum_k vA[k] * x[JA[k]] over every row; empty rows give 0.
+void spmv(const Csr &A, const double *x, double *y);
+void fill(double *y, int64_t n, double v);
+int64_t exclusive_scan_i32(const int32_t *d_counts, int64_t m, int32_t *d_out);
+int lds_per_block();
+void row_norms(const Csr &A, int p, double *out);  // (sum |a|^p)^(1/p), pow-based
+
+// ---- wavefront reduction --------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Fortran MIN / MAX on reals as the reference compiler evaluates them:
+// min(a,b) = a<b ? a : b, max(a,b) = a>b ? a : b (a NaN `a` yields b).
+__host__ __device__ __forceinline__ double fmin_ref(double a, double b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ double fmax_ref(double a, double b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ float fmin_ref(float a, float b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ float fmax_ref(float a, float b) { return a > b ? a : b; }
+
+inline unsigned grid_for(int64_t items, int per_block) {
+    int64_t g = (items + per_block - 1) / per_block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace fair

@@ -1,0 +1,618 @@
+// methods.hip -- the AIRMs methods on the device: SIRT (Cimmino, SART), normest, gradient
+// descent and the ART family (Kaczmarz, symmetric, randomized) as a persistent sweep.
+//
+// SIRT / GD iterate as two fused launches each, all state resident in HBM:
+//   launch 1: row-per-wavefront A*x whose epilogue forms the weighted residual
+//             (SART/Cimmino: r = (b - Ax)/M ; GD: r = Ax - b)
+//   launch 2: row-per-wavefront A^T*r over the device-built CSR of A^T whose epilogue applies
+//             the update and the box (SIRT: x + relax*(A^T r)/D ; GD: x - rho/sigma^2 A^T r).
+// Gradient descent's stopping rule (||x_new - x|| < tol, AIRMs.f90:972-975) is decided on the
+// device: a one-block reduction sets a done flag that gates every later launch, so the whole
+// loop is enqueued without host round trips.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fair_internal.hpp"
+#include "spmv.hpp"
+
+namespace fair {
+namespace {
+
+struct Box {
+    int upb, lwrb;
+    double ub, lb;
+    // AIRMs.f90:1093-1100 / :963-970: min(ub) first, then max(lb)
+    __device__ __forceinline__ double apply(double v) const {
+        if (upb) v = fmin_ref(v, ub);
+        if (lwrb) v = fmax_ref(v, lb);
+        return v;
+    }
+};
+
+Box make_box(const fair_opts *o) {
+    Box b{0, 0, 0.0, 1.0};
+    if (o) {
+        b.upb = o->upb != 0;
+        b.lwrb = o->lwrb != 0;
+        b.ub = o->ubound;
+        b.lb = o->lbound;
+    }
+    return b;
+}
+
+// r = (b - Ax) / M   (AIRMs.f90:1090, inner part)
+struct EpiResidual {
+    const double *b, *M;
+    double *r;
+    __device__ __forceinline__ void operator()(int32_t i, double acc) const { r[i] = (b[i] - acc) / M[i]; }
+};
+
+// x_new = box(x + (relax * A^T r) / D)   (AIRMs.f90:1090 outer part + :1093-1100)
+struct EpiSirtUpdate {
+    const double *x, *D;   // D == nullptr: Cimmino's D = 1
+    double *xn;
+    double relax;
+    Box box;
+    __device__ __forceinline__ void operator()(int32_t j, double acc) const {
+        double u = relax * acc;
+        if (D) u = u / D[j];
+        xn[j] = box.apply(x[j] + u);
+    }
+};
+
+__global__ void k_cimmino_weights(double *M, int32_t m) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) M[i] = (double)m * (M[i] * M[i]);   // AIRMs.f90:1072: m * (row_norms(2))**2
+}
+
+__global__ void k_copy(const double *a, double *b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__global__ void k_scale_div(double *v, int64_t n, double d) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = v[i] / d;
+}
+
+__global__ void k_neg(const double *a, double *b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = -a[i];
+}
+
+// ---- deterministic reductions -----------------------------------------------------------
+// Fixed-shape two-level sum: each of RED_BLOCKS blocks sums a strided slice in a fixed order,
+// one block adds the partials.  Same inputs -> same bits.
+constexpr int RED_BLOCKS = 512;
+
+template <class F>
+__global__ void __launch_bounds__(256) k_red_partial(int64_t n, F f, double *part) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * 256)
+        s += f(i);
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__device__ double block_sum_parts(const double *part) {
+    __shared__ double sh[8];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < RED_BLOCKS; i += 512) s += part[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < 8; ++w) t += sh[w];
+    return t;
+}
+
+__global__ void __launch_bounds__(512) k_red_final(const double *part, double *out) {
+    double t = block_sum_parts(part);
+    if (threadIdx.x == 0) *out = t;
+}
+
+struct SqOf {
+    const double *a;
+    __device__ __forceinline__ double operator()(int64_t i) const { return a[i] * a[i]; }
+};
+struct SqDiff {
+    const double *a, *b;
+    __device__ __forceinline__ double operator()(int64_t i) const {
+        double d = a[i] - b[i];
+        return d * d;
+    }
+};
+
+template <class F>
+double dev_sum(int64_t n, F f, DBuf<double> &part, DBuf<double> &out) {
+    hipLaunchKernelGGL(k_red_partial<F>, dim3(RED_BLOCKS), dim3(256), 0, stream(), n, f, part.p);
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(512), 0, stream(), part.p, out.p);
+    launch_check("reduction");
+    double h = 0.0;
+    out.download(&h, 1);
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    return h;
+}
+
+// ---- gradient descent with a device-side stopping flag ------------------------------------
+struct GdState {
+    int32_t iter;   // accepted updates so far; the current iterate is xbuf[iter & 1]
+    int32_t done;
+};
+
+struct SrcGated {
+    const GdState *st;
+    const double *x;
+    __device__ __forceinline__ const double *operator()() const { return st->done ? nullptr : x; }
+};
+struct SrcParity {
+    const GdState *st;
+    const double *x0, *x1;
+    __device__ __forceinline__ const double *operator()() const {
+        return st->done ? nullptr : ((st->iter & 1) ? x1 : x0);
+    }
+};
+
+// xn = box(x - step * A^T r); d2 = (xn - x)^2   (AIRMs.f90:960-970)
+struct EpiGdUpdate {
+    const GdState *st;
+    double *x0, *x1, *d2;
+    double step;
+    Box box;
+    __device__ __forceinline__ void operator()(int32_t j, double acc) const {
+        const bool odd = st->iter & 1;
+        const double *cur = odd ? x1 : x0;
+        double *nxt = odd ? x0 : x1;
+        double v = box.apply(cur[j] - step * acc);
+        nxt[j] = v;
+        double d = v - cur[j];
+        d2[j] = d * d;
+    }
+};
+
+// r = A x - b   (AIRMs.f90:979)
+struct EpiGdResidual {
+    const double *b;
+    double *r;
+    __device__ __forceinline__ void operator()(int32_t i, double acc) const { r[i] = acc - b[i]; }
+};
+
+struct D2Of {
+    const double *d2;
+    __device__ __forceinline__ double operator()(int64_t i) const { return d2[i]; }
+};
+
+__global__ void __launch_bounds__(256) k_gd_partial(int64_t n, const double *d2, GdState *st, double *part) {
+    if (st->done) return;
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * 256)
+        s += d2[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ void __launch_bounds__(512) k_gd_decide(const double *part, GdState *st, double tol) {
+    if (st->done) return;
+    double t = block_sum_parts(part);
+    if (threadIdx.x == 0) {
+        if (sqrt(t) < tol) st->done = 1;
+        else st->iter += 1;
+    }
+}
+
+// ---- ART: persistent row-ordered sweep ------------------------------------------------------
+// One workgroup walks the row sequence strictly in order.  Per row: block-wide dot a_i.x
+// (AIRMs.f90:841), coeff = relax*(b_i - dot)/nrA_i (:842), then x(JA) += coeff*vA with
+// max(lb) / min(ub) on the touched entries (sparse_matrices.f90:808-820).  Column indices are
+// unique within a row, so each entry has one owner thread; the barrier after the update
+// orders it before the next row's dot.
+constexpr int SWEEP_THREADS = 512;
+
+__global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(const int32_t *__restrict__ rows, int64_t nsteps,
+                                                             const int32_t *__restrict__ IA,
+                                                             const int32_t *__restrict__ JA,
+                                                             const double *__restrict__ vA,
+                                                             const double *__restrict__ b,
+                                                             const double *__restrict__ nrA,
+                                                             int32_t ncols, double relax, int lwrb,
+                                                             double lb, int upb, double ub, double *x) {
+    __shared__ double part[SWEEP_THREADS / WAVE];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        const int r = rows[s] - 1;
+        const int32_t beg = IA[r], end = IA[r + 1];
+        double dot = 0.0;
+        for (int32_t k = beg + tid; k < end; k += SWEEP_THREADS)
+            if ((uint32_t)JA[k] < (uint32_t)ncols) dot += x[JA[k]] * vA[k];
+        dot = wave_sum(dot);
+        if (lane == 0) part[wv] = dot;
+        __syncthreads();
+        double inp = 0.0;
+#pragma unroll
+        for (int w = 0; w < SWEEP_THREADS / WAVE; ++w) inp += part[w];
+        const double coeff = (relax * (b[r] - inp)) / nrA[r];
+        for (int32_t k = beg + tid; k < end; k += SWEEP_THREADS) {
+            const int j = JA[k];
+            if ((uint32_t)j >= (uint32_t)ncols) continue;
+            double v = x[j] + coeff * vA[k];
+            if (lwrb) v = fmax_ref(v, lb);
+            if (upb) v = fmin_ref(v, ub);
+            x[j] = v;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_art_weights(double *nrA, int32_t m, double add) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) nrA[i] = nrA[i] + add;
+}
+
+__global__ void k_square(double *v, int32_t m) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) v[i] = v[i] * v[i];
+}
+
+void check_handle(const fair_csr *A) { FAIR_CHECK(A != nullptr, FAIR_ERR_ARG, "null matrix handle"); }
+
+// SIRT weights (AIRMs.f90:1070-1083), computed once per matrix and cached in the handle.
+// Cimmino: M = m*||a_i||^2, D = 1.  SART: M = row sums, then the reference's double transpose
+// (which leaves A's rows column-sorted), D = column sums.
+void sirt_setup(fair_csr *h, int method) {
+    FAIR_CHECK(method == FAIR_CIMMINO || method == FAIR_SART, FAIR_ERR_METHOD, "Not a recognised SIRT method.");
+    Csr &A = h->A;
+    if (A.n_rows_nz < 0) build_row_list(A);
+    const int32_t m = A.m, n = A.n;
+    if (!h->w_ok[method]) {
+        h->M[method].alloc(m > 0 ? m : 1);
+        if (method == FAIR_CIMMINO) {
+            row_norms(A, 2, h->M[method].p);
+            hipLaunchKernelGGL(k_cimmino_weights, dim3(grid_for(m, 256)), dim3(256), 0, stream(), h->M[method].p, m);
+        } else {
+            row_norms(A, 1, h->M[method].p);          // AIRMs.f90:1076
+            if (!h->rows_sorted) {                    // :1077-1079
+                sort_rows(A);
+                h->rows_sorted = true;
+            }
+            ensure_T(h);
+            h->D.alloc(n > 0 ? n : 1);
+            row_norms(h->T, 1, h->D.p);               // column sums
+        }
+        launch_check("sirt weights");
+        h->w_ok[method] = true;
+    }
+    ensure_T(h);
+    if (h->ws_r.n < (size_t)(m > 0 ? m : 1)) h->ws_r.alloc(m > 0 ? m : 1);
+    if (h->ws_x.n < (size_t)(n > 0 ? n : 1)) h->ws_x.alloc(n > 0 ? n : 1);
+}
+
+// maxits SIRT iterations on d_x (in: start, out: result), two launches per iteration.
+void sirt_iterate(fair_csr *h, int method, const double *d_b, int maxits, double *d_x, const fair_opts *o) {
+    const Csr &A = h->A, &T = h->T;
+    const int32_t n = A.n;
+    const double *M = h->M[method].p;
+    const double *D = method == FAIR_SART ? h->D.p : nullptr;
+    const double relax = o ? o->relaxpar : 1.0;
+    const Box box = make_box(o);
+    double *r = h->ws_r.p;
+    double *cur = d_x, *nxt = h->ws_x.p;
+    for (int it = 0; it < maxits; ++it) {
+        spmv_launch(A, cur, EpiResidual{d_b, M, r});
+        EpiSirtUpdate up{cur, D, nxt, relax, box};
+        spmv_launch(T, r, up);
+        if (T.n_rows_empty > 0)
+            hipLaunchKernelGGL((k_empty_rows<double, SrcPtr<double>, EpiSirtUpdate>),
+                               dim3(grid_for(T.n_rows_empty, 256)), dim3(256), 0, stream(),
+                               T.rows_empty.p, T.n_rows_empty, SrcPtr<double>{r}, up);
+        std::swap(cur, nxt);
+    }
+    if (cur != d_x)
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256)), dim3(256), 0, stream(), cur, d_x, (int64_t)n);
+    launch_check("sirt");
+}
+
+void sirt_run(fair_csr *h, int method, const double *d_b, int maxits, double *d_x, const fair_opts *o) {
+    sirt_setup(h, method);
+    sirt_iterate(h, method, d_b, maxits, d_x, o);
+}
+
+}  // namespace
+}  // namespace fair
+
+using namespace fair;
+
+extern "C" int fair_sirt_dev(fair_csr *h, int32_t method, const double *d_b, int32_t maxits, double *d_x,
+                             const fair_opts *opts) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(maxits >= 0, FAIR_ERR_ARG, "negative iteration count");
+        sirt_run(h, method, d_b, maxits, d_x, opts);
+    });
+}
+
+extern "C" int fair_sirt_prepare(fair_csr *h, int32_t method) {
+    return guard([&] {
+        check_handle(h);
+        sirt_setup(h, method);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+    });
+}
+
+extern "C" int fair_sirt(fair_csr *h, int32_t method, const double *b, int32_t maxits, const double *x0,
+                         const fair_opts *opts, double *xout) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(maxits >= 0, FAIR_ERR_ARG, "negative iteration count");
+        FAIR_CHECK(b && xout, FAIR_ERR_ARG, "null argument");
+        const int32_t m = h->A.m, n = h->A.n;
+        DBuf<double> db(m > 0 ? m : 1), dx(n > 0 ? n : 1);
+        db.upload(b, m);
+        if (x0) dx.upload(x0, n);
+        else fill(dx.p, n, 0.0);
+        sirt_run(h, method, db.p, maxits, dx.p, opts);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        dx.download(xout, n);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+    });
+}
+
+extern "C" int fair_normest(fair_csr *h, const double *v0, double tol, int32_t maxit, double *sigma) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(v0 && sigma, FAIR_ERR_ARG, "null argument");
+        Csr &A = h->A;
+        if (A.n_rows_nz < 0) build_row_list(A);
+        ensure_T(h);
+        const int32_t m = A.m, n = A.n;
+        DBuf<double> v(n > 0 ? n : 1), vn(n > 0 ? n : 1), w(m > 0 ? m : 1), part(RED_BLOCKS), out(1);
+        v.upload(v0, n);
+        // AIRMs.f90:1159-1167
+        for (int i = 1; i <= maxit; ++i) {
+            spmv(A, v.p, w.p);
+            spmv(h->T, w.p, vn.p);
+            double nv = std::sqrt(dev_sum(n, SqOf{vn.p}, part, out));
+            hipLaunchKernelGGL(k_scale_div, dim3(grid_for(n, 256)), dim3(256), 0, stream(), vn.p, (int64_t)n, nv);
+            double dd = std::sqrt(dev_sum(n, SqDiff{vn.p, v.p}, part, out));
+            if (dd < tol) break;
+            std::swap(v.p, vn.p);
+        }
+        spmv(A, v.p, w.p);                                   // :1169-1171
+        double ww = dev_sum(m, SqOf{w.p}, part, out);
+        double vv = dev_sum(n, SqOf{v.p}, part, out);
+        *sigma = std::sqrt(ww / vv);
+    });
+}
+
+extern "C" int fair_gradient_descent(fair_csr *h, const double *b, int32_t maxiter, const double *x0,
+                                     double rho, double tol, double sigma, const fair_opts *opts,
+                                     double *xout, int32_t *iters) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(b && xout, FAIR_ERR_ARG, "null argument");
+        FAIR_CHECK(tol >= 0.0, FAIR_ERR_ARG, "Must use positive tolerance.");
+        Csr &A = h->A;
+        if (A.n_rows_nz < 0) build_row_list(A);
+        ensure_T(h);
+        const Csr &T = h->T;
+        const int32_t m = A.m, n = A.n;
+        DBuf<double> db(m > 0 ? m : 1), r(m > 0 ? m : 1), x0b(n > 0 ? n : 1), x1b(n > 0 ? n : 1),
+            d2(n > 0 ? n : 1), part(RED_BLOCKS);
+        DBuf<GdState> st(1);
+        db.upload(b, m);
+        if (x0) x0b.upload(x0, n);
+        else fill(x0b.p, n, 0.0);
+        GdState s0{0, 0};
+        st.upload(&s0, 1);
+        const double s2 = sigma * sigma;                     // AIRMs.f90:953
+        const double step = rho / s2;                        // rho/sigma * A^T r, :960
+        const Box box = make_box(opts);
+        if (m > 0)
+            hipLaunchKernelGGL(k_neg, dim3(grid_for(m, 256)), dim3(256), 0, stream(), db.p, r.p, (int64_t)m);
+        for (int it = 1; it <= maxiter; ++it) {
+            EpiGdUpdate up{st.p, x0b.p, x1b.p, d2.p, step, box};
+            SrcGated src{st.p, r.p};
+            spmv_launch_src(T, src, up);
+            if (T.n_rows_empty > 0)
+                hipLaunchKernelGGL((k_empty_rows<double, SrcGated, EpiGdUpdate>),
+                                   dim3(grid_for(T.n_rows_empty, 256)), dim3(256), 0, stream(),
+                                   T.rows_empty.p, T.n_rows_empty, src, up);
+            hipLaunchKernelGGL(k_gd_partial, dim3(RED_BLOCKS), dim3(256), 0, stream(), (int64_t)n, d2.p, st.p, part.p);
+            hipLaunchKernelGGL(k_gd_decide, dim3(1), dim3(512), 0, stream(), part.p, st.p, tol);
+            if (it < maxiter)
+                spmv_launch_src(A, SrcParity{st.p, x0b.p, x1b.p}, EpiGdResidual{db.p, r.p});
+        }
+        launch_check("gradient descent");
+        GdState fin;
+        st.download(&fin, 1);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        ((fin.iter & 1) ? x1b : x0b).download(xout, n);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        if (iters) *iters = fin.iter;
+    });
+}
+
+// ---- ART --------------------------------------------------------------------------------------
+namespace {
+
+struct ArtPlan {
+    int method;
+    std::vector<int32_t> Kuse, ord;
+    int nord = 0;
+    bool isrand = false, rowprob = false;
+    std::vector<double> nrA, cumul;
+    int maxiter = 0;
+};
+
+// Ordering, damping, row filter and row-probability table of art (AIRMs.f90:217-312).
+ArtPlan plan_art(fair_csr *h, int method, int nk, const int32_t *Kin, const int32_t *order, int nord_in,
+                 const fair_opts *o, DBuf<double> &d_nrA) {
+    FAIR_CHECK(method >= FAIR_KACZMARZ && method <= FAIR_RANDKACZMARZ, FAIR_ERR_METHOD,
+               "Unknown ART method. Use kaczmarz, symkaczmarz or randkaczmarz");
+    FAIR_CHECK(nk >= 1 && Kin, FAIR_ERR_ARG, "Kin must not be empty");
+    Csr &A = h->A;
+    if (A.n_rows_nz < 0) build_row_list(A);
+    const int32_t m = A.m;
+    ArtPlan P;
+    P.method = method;
+    P.isrand = method == FAIR_RANDKACZMARZ;
+    P.rowprob = P.isrand && o && o->rowprobability;
+    P.Kuse.assign(Kin, Kin + nk);
+    if (method == FAIR_SYMKACZMARZ) {
+        for (int k = 0; k < nk; ++k) FAIR_CHECK(Kin[k] % 2 == 0, FAIR_ERR_ARG, "Kin should all be multiples of 2.");
+        for (auto &k : P.Kuse) k /= 2;
+    }
+    std::vector<int32_t> ord;
+    if (order) {
+        FAIR_CHECK(nord_in >= 0 && nord_in <= m, FAIR_ERR_ARG, "order longer than the number of rows");
+        ord.assign(order, order + nord_in);
+        for (int32_t r : ord) FAIR_CHECK(r >= 1 && r <= m, FAIR_ERR_ARG, "order entry out of range");
+    } else {
+        ord.resize(m);
+        for (int32_t i = 0; i < m; ++i) ord[i] = i + 1;
+    }
+    // nrA = row_norms(2)**2 + damp*maxval (AIRMs.f90:273-275)
+    d_nrA.alloc(m > 0 ? m : 1);
+    row_norms(A, 2, d_nrA.p);
+    if (m > 0) hipLaunchKernelGGL(k_square, dim3(grid_for(m, 256)), dim3(256), 0, stream(), d_nrA.p, m);
+    P.nrA.resize(m);
+    d_nrA.download(P.nrA.data(), m);
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    const double dmp = o ? o->damp : 0.0;
+    if (dmp != 0.0 && m > 0) {
+        double mx = P.nrA[0];
+        for (double v : P.nrA) mx = v > mx ? v : mx;
+        const double add = dmp * mx;
+        for (double &v : P.nrA) v = v + add;
+        d_nrA.upload(P.nrA.data(), m);
+    }
+    // keep rows with nrA > 0 (:279-287)
+    std::vector<int32_t> kept;
+    kept.reserve(ord.size());
+    for (int32_t r : ord)
+        if (P.nrA[r - 1] > 0) kept.push_back(r);
+    P.nord = (int)kept.size();
+    P.ord = kept;
+    if (P.rowprob) {                                   // :302-308
+        P.cumul.assign(m, 1.0);
+        double cs = 0.0, tot = 0.0;
+        for (int i = 0; i < P.nord; ++i) { cs = cs + P.nrA[i]; P.cumul[i] = cs; }
+        for (int i = 0; i < m; ++i) tot = tot + P.nrA[i];
+        for (int i = 0; i < P.nord; ++i) P.cumul[i] = P.cumul[i] / tot;
+    }
+    P.maxiter = 0;
+    for (int k : P.Kuse) P.maxiter = std::max(P.maxiter, k);
+    return P;
+}
+
+int64_t uniforms_needed(const ArtPlan &P) {
+    if (!P.isrand) return 0;
+    return 2 * (int64_t)P.nord + (int64_t)P.maxiter * (2 * (int64_t)P.nord + (P.rowprob ? P.nord : 0));
+}
+
+// scramble (AIRMs.f90:1175-1212): two passes of "swap i with a uniform j", values 1..nord
+void scramble(int nord, const double *u, int32_t *out) {
+    for (int i = 0; i < nord; ++i) out[i] = i + 1;
+    int64_t q = 0;
+    for (int k = 0; k < 2; ++k)
+        for (int i = 1; i <= nord; ++i) {
+            int j = 1 + (int)std::floor((double)nord * u[q++]);
+            std::swap(out[j - 1], out[i - 1]);
+        }
+}
+
+}  // namespace
+
+extern "C" int fair_art_uniform_count(fair_csr *h, int32_t method, int32_t nk, const int32_t *Kin,
+                                      const int32_t *order, int32_t nord, const fair_opts *opts,
+                                      int64_t *count) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(count != nullptr, FAIR_ERR_ARG, "null argument");
+        DBuf<double> d_nrA;
+        ArtPlan P = plan_art(h, method, nk, Kin, order, nord, opts, d_nrA);
+        *count = uniforms_needed(P);
+    });
+}
+
+extern "C" int fair_art_steps_dev(fair_csr *h, const int32_t *d_rows, int64_t nsteps, const double *d_b,
+                                  const double *d_nrA, const fair_opts *o, double *d_x) {
+    return guard([&] {
+        check_handle(h);
+        const Box box = make_box(o);
+        const double relax = o ? o->relaxpar : 1.0;
+        if (nsteps > 0)
+            hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(), d_rows, nsteps,
+                               h->A.IA.p, h->A.JA.p, h->A.vA.p, d_b, d_nrA, h->A.n, relax, box.lwrb,
+                               box.lb, box.upb, box.ub, d_x);
+        launch_check("art sweep");
+    });
+}
+
+extern "C" int fair_art(fair_csr *h, int32_t method, const double *b, int32_t nk, const int32_t *Kin,
+                        const double *x0, const int32_t *order, int32_t nord, const fair_opts *opts,
+                        const double *uniforms, int64_t nu, double *Xout) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(b && Xout, FAIR_ERR_ARG, "null argument");
+        DBuf<double> d_nrA;
+        ArtPlan P = plan_art(h, method, nk, Kin, order, nord, opts, d_nrA);
+        const int64_t need = uniforms_needed(P);
+        FAIR_CHECK(need <= nu && (need == 0 || uniforms), FAIR_ERR_ARG,
+                   "too few uniforms: need " + std::to_string(need));
+        const int32_t m = h->A.m, n = h->A.n;
+        const Box box = make_box(opts);
+        const double relax = opts ? opts->relaxpar : 1.0;
+        DBuf<double> db(m > 0 ? m : 1), dx(n > 0 ? n : 1);
+        db.upload(b, m);
+        if (x0) dx.upload(x0, n);
+        else fill(dx.p, n, 0.0);
+        const int steps_per_iter = P.method == FAIR_SYMKACZMARZ ? 2 * P.nord : P.nord;
+        std::vector<int32_t> seq(std::max(1, steps_per_iter));
+        DBuf<int32_t> dseq(seq.size());
+        std::vector<int32_t> ord = P.ord;
+        int64_t q = 0;
+        if (P.isrand) {                                   // :297
+            scramble(P.nord, uniforms + q, ord.data());
+            q += 2 * (int64_t)P.nord;
+        }
+        std::vector<double> snap(n);
+        size_t rec = 0;
+        for (int iter = 1; iter <= P.maxiter; ++iter) {
+            if (P.isrand) {                               // :320
+                scramble(P.nord, uniforms + q, ord.data());
+                q += 2 * (int64_t)P.nord;
+            }
+            for (int ii = 0; ii < P.nord; ++ii) {
+                if (P.rowprob) {                          // :828-835 (first ri with cumul(ri) >= u)
+                    double u = uniforms[q++];
+                    auto it = std::lower_bound(P.cumul.begin(), P.cumul.end(), u);
+                    int ri = (int)(it - P.cumul.begin()) + 1;
+                    seq[ii] = std::min(ri, (int)m);
+                } else {
+                    seq[ii] = ord[ii];
+                }
+            }
+            if (P.method == FAIR_SYMKACZMARZ)             // :326-328
+                for (int ii = 0; ii < P.nord; ++ii) seq[P.nord + ii] = ord[P.nord - 1 - ii];
+            if (steps_per_iter > 0) {
+                dseq.upload(seq.data(), steps_per_iter);
+                hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(), dseq.p,
+                                   (int64_t)steps_per_iter, h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p,
+                                   h->A.n, relax, box.lwrb, box.lb, box.upb, box.ub, dx.p);
+                launch_check("art sweep");
+            }
+            if (rec < P.Kuse.size() && iter == P.Kuse[rec]) {   // :337-340
+                dx.download(Xout + rec * (size_t)n, n);
+                ++rec;
+            }
+            FAIR_HIP(hipStreamSynchronize(stream()));
+        }
+    });
+}

@@ -1,0 +1,149 @@
+/*
+ * fair_hip.h -- C ABI of the MI355X (gfx950) FAIR tools reconstruction path.
+ *
+ * One process, one GPU.  Every entry point returns a status (FAIR_OK = 0); the text of the
+ * last error of the calling thread is available from fair_last_error().  Host arrays follow
+ * the reference's conventions: IA holds m+1 zero-based row offsets, JA holds one-based
+ * column numbers, vectors are contiguous double (or float for the _f32 variants).
+ * Functions named *_dev take device pointers (inputs already resident in HBM) and run on the
+ * stream set by fair_set_stream (default: the null stream); all others take host arrays and
+ * return after the result has been copied back.
+ *
+ * Each declaration cites the reference procedure it replaces (file:line in FAIR tools,
+ * BartvLith/fortran_AIRtools).
+ */
+#ifndef FAIR_HIP_H
+#define FAIR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FAIR_OK = 0,
+    FAIR_ERR_ARG = 1,       /* bad argument or size mismatch ("Vector input size incorrect.") */
+    FAIR_ERR_HIP = 2,       /* HIP runtime failure (allocation, launch, copy) */
+    FAIR_ERR_OVERFLOW = 3,  /* nnz or an offset does not fit the reference's int32 */
+    FAIR_ERR_INTERNAL = 4,  /* consistency check failed ("++ Something went wrong") */
+    FAIR_ERR_METHOD = 5     /* unknown method name / invalid method arguments */
+};
+
+/* Geometry record, bind(C) mirror of `type(data)` (system_generation.f90:6-12) without the
+ * allocatable arrays; theta (radians) is passed separately. */
+typedef struct fair_geom {
+    int32_t n, p, nth;
+    double r, dw, sd;
+} fair_geom;
+
+/* Method options, bind(C) mirror of `type(opts)` (AIRMs.f90:6-13). */
+typedef struct fair_opts {
+    double relaxpar, ubound, lbound, damp;
+    int32_t verbose, upb, lwrb, rowprobability;
+} fair_opts;
+
+typedef struct fair_csr fair_csr; /* opaque device CSR matrix (+ cached CSR of its transpose) */
+
+/* ---- runtime ------------------------------------------------------------------------ */
+const char *fair_last_error(void);
+/* Select the device and prepare the library (no counterpart: the reference has no device). */
+int fair_init(int device);
+/* Stream for all subsequent launches (hipStream_t passed as void*; NULL = null stream). */
+int fair_set_stream(void *stream);
+int fair_synchronize(void);
+/* Library version string. */
+const char *fair_version(void);
+
+/* ---- system_generation -------------------------------------------------------------- */
+/* fanlineartomo, system_generation.f90:79-307 (count -> scan -> fill on the device). */
+int fair_fanlineartomo(const fair_geom *g, const double *theta, fair_csr **A);
+/* paralleltomo, system_generation.f90:309-492. */
+int fair_paralleltomo(const fair_geom *g, const double *theta, fair_csr **A);
+/* linspace, system_generation.f90:17-35 (host). */
+int fair_linspace(double a, double b, int32_t n, double *x);
+
+/* ---- sparse_matrices: construction and transfer (type(sparse_matrix), :33-176) -------- */
+/* Upload a host CSR (the components m, n, nnz, vA, IA(0:m), JA of sparse_matrices.f90:47-49). */
+int fair_csr_create(int32_t m, int32_t n, int64_t nnz, const int32_t *IA, const int32_t *JA,
+                    const double *vA, fair_csr **A);
+int fair_csr_info(const fair_csr *A, int32_t *m, int32_t *n, int64_t *nnz);
+/* Download the components (JA one-based). */
+int fair_csr_download(const fair_csr *A, int32_t *IA, int32_t *JA, double *vA);
+/* Device addresses of IA (m+1, int32), zero-based JA (int32) and vA (double). */
+int fair_csr_device_arrays(const fair_csr *A, const int32_t **IA, const int32_t **JA0,
+                           const double **vA);
+int fair_csr_free(fair_csr *A);
+
+/* ---- sparse_matrices: operations ---------------------------------------------------- */
+/* matvecmul -> sparsematmul/MMRS, sparse_matrices.f90:272-298, 349-368. */
+int fair_matvecmul(fair_csr *A, const double *v, int32_t nv, double *w, int32_t nw);
+/* transvecmul -> sparsetransmul/MMCS, sparse_matrices.f90:300-347, 370-393 (gather over the
+ * device-built CSR of A^T: deterministic, no scatter race). */
+int fair_transvecmul(fair_csr *A, const double *v, int32_t nv, double *w, int32_t nw);
+/* rowvecmult -> csr_matrowvec, sparse_matrices.f90:252-270 (k one-based). */
+int fair_rowvecmult(fair_csr *A, int32_t k, const double *v, int32_t nv, double *f);
+/* row_norms -> compute_row_norms, sparse_matrices.f90:740-780 (p >= 1). */
+int fair_row_norms(fair_csr *A, int32_t p, double *nrA, int32_t nm);
+/* column_norms -> compute_column_norms, sparse_matrices.f90:782-793. */
+int fair_column_norms(fair_csr *A, int32_t p, double *ncA, int32_t nn);
+/* transpose -> transposition/reorder, sparse_matrices.f90:395-411, 492-570 (in place; the
+ * result's rows are sorted by column, as the counting sort leaves them). */
+int fair_transpose(fair_csr *A);
+/* multiply_by_scalar -> scalar_mult, sparse_matrices.f90:824-831. */
+int fair_multiply_by_scalar(fair_csr *A, double c);
+/* add_scalar_mult_row -> add_mult_row, sparse_matrices.f90:795-822 (k one-based; lwrb/upb
+ * NULL when absent). */
+int fair_add_scalar_mult_row(fair_csr *A, int32_t k, double c, double *w, int32_t nw,
+                             const double *lwrb, const double *upb);
+
+/* Device-pointer forms of matvecmul / transvecmul (x, y resident in HBM). */
+int fair_matvecmul_dev(fair_csr *A, const double *d_v, double *d_w);
+int fair_transvecmul_dev(fair_csr *A, const double *d_v, double *d_w);
+/* MMRS (sparse_matrices.f90:349-368) restricted to the non-empty rows: one kernel launch; the
+ * entries of w belonging to empty rows are left untouched (callers zero them once). */
+int fair_spmv_rows_dev(fair_csr *A, const double *d_v, double *d_w);
+
+/* ---- AIRMs ----------------------------------------------------------------------------- */
+enum { FAIR_CIMMINO = 0, FAIR_SART = 1 };
+enum { FAIR_KACZMARZ = 0, FAIR_SYMKACZMARZ = 1, FAIR_RANDKACZMARZ = 2 };
+
+/* sirt, AIRMs.f90:997-1127: maxits = maxval(Kin) iterations of
+ * x <- box(x + relax * D^-1 A^T (M^-1 (b - A x))).  x0 may be NULL (zero start).  As in the
+ * reference, SART leaves A's rows sorted by column. */
+int fair_sirt(fair_csr *A, int32_t method, const double *b, int32_t maxits, const double *x0,
+              const fair_opts *opts, double *xout);
+/* Device-resident form: b (m), x (n, in: start, out: result) in HBM.  Asynchronous on the
+ * library stream once the weights exist. */
+int fair_sirt_dev(fair_csr *A, int32_t method, const double *d_b, int32_t maxits, double *d_x,
+                  const fair_opts *opts);
+/* Compute and cache the SIRT weights of sirt (AIRMs.f90:1070-1083) for A (SART also re-sorts
+ * A's rows as the reference's double transpose does), so later fair_sirt_dev calls only
+ * iterate. */
+int fair_sirt_prepare(fair_csr *A, int32_t method);
+/* normest, AIRMs.f90:1129-1173, started from v0 (the reference draws it with random_number). */
+int fair_normest(fair_csr *A, const double *v0, double tol, int32_t maxit, double *sigma);
+/* gradient_descent, AIRMs.f90:860-995.  sigma is normest's result (before squaring); iters
+ * receives the number of accepted updates. */
+int fair_gradient_descent(fair_csr *A, const double *b, int32_t maxiter, const double *x0,
+                          double rho, double tol, double sigma, const fair_opts *opts,
+                          double *xout, int32_t *iters);
+/* art, AIRMs.f90:59-369 (without the xref early stop).  order may be NULL (natural order).
+ * uniforms: the random_number draws in the reference's order (see fair_art_uniform_count);
+ * may be NULL for the non-random methods.  Xout: nk x n, row k = iterate at Kin[k]. */
+int fair_art(fair_csr *A, int32_t method, const double *b, int32_t nk, const int32_t *Kin,
+             const double *x0, const int32_t *order, int32_t nord, const fair_opts *opts,
+             const double *uniforms, int64_t nu, double *Xout);
+/* Number of random_number draws art consumes for these arguments. */
+int fair_art_uniform_count(fair_csr *A, int32_t method, int32_t nk, const int32_t *Kin,
+                           const int32_t *order, int32_t nord, const fair_opts *opts,
+                           int64_t *count);
+/* basic_sweep's row steps, AIRMs.f90:840-856, over an explicit one-based row sequence, as one
+ * persistent launch.  nrA: damped squared row norms (m).  d_x in HBM. */
+int fair_art_steps_dev(fair_csr *A, const int32_t *d_rows, int64_t nsteps, const double *d_b,
+                       const double *d_nrA, const fair_opts *opts, double *d_x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FAIR_HIP_H */

@@ -1,0 +1,319 @@
+! oracle/ref_harness.f90 -- C-callable driver around the reference modules.
+!
+! TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile together with the reference's own
+! sparse_matrices / system_generation / AIRMs / phantomgallery sources (compiled where they
+! lie, never copied) into oracle/_ref/libfair_ref.so.  Golden-vector scripts under
+! tests/golden/ and bench.py's CPU-baseline leg load it; the product never does.
+!
+! Matrices live in module-level slots so that Python can drive the reference API
+! (fanlineartomo, paralleltomo, matvecmul, transvecmul, row_norms, transpose, sirt,
+! gradient_descent, normest, art) exactly as a Fortran caller would.
+module ref_harness
+    use iso_c_binding
+    use sparse_matrices
+    use system_generation
+    use airms
+    use phantomgallery
+    implicit none
+    integer, parameter :: NSLOT = 8
+    type(csr_matrix), save :: mats(NSLOT)
+contains
+
+    subroutine make_geom(dat, n, p, nth, r, dw, sd, theta)
+        type(data), intent(out) :: dat
+        integer(c_int), value :: n, p, nth
+        real(c_double), value :: r, dw, sd
+        real(c_double), intent(in) :: theta(nth)
+        dat%n = n
+        dat%p = p
+        dat%nth = nth
+        dat%r = r
+        dat%dw = dw
+        dat%sd = sd
+        allocate (dat%theta(nth))
+        dat%theta = theta
+    end subroutine make_geom
+
+    ! kind 0 = fanlineartomo, 1 = paralleltomo.  Returns nnz.
+    function ref_build(slot, kind, n, p, nth, r, dw, sd, theta) bind(C, name='ref_build') result(nnz)
+        integer(c_int), value :: slot, kind, n, p, nth
+        real(c_double), value :: r, dw, sd
+        real(c_double), intent(in) :: theta(nth)
+        integer(c_int) :: nnz
+        type(data) :: dat
+        type(csr_matrix) :: tmp
+        call make_geom(dat, n, p, nth, r, dw, sd, theta)
+        if (kind == 0) then
+            tmp = fanlineartomo(dat)
+        else
+            tmp = paralleltomo(dat)
+        end if
+        call take(slot, tmp)
+        nnz = mats(slot)%nnz
+    end function ref_build
+
+    subroutine take(slot, tmp)
+        integer(c_int), value :: slot
+        type(csr_matrix), intent(inout) :: tmp
+        if (allocated(mats(slot)%IA)) deallocate (mats(slot)%IA)
+        if (allocated(mats(slot)%JA)) deallocate (mats(slot)%JA)
+        if (allocated(mats(slot)%vA)) deallocate (mats(slot)%vA)
+        mats(slot)%m = tmp%m
+        mats(slot)%n = tmp%n
+        mats(slot)%nnz = tmp%nnz
+        call move_alloc(tmp%IA, mats(slot)%IA)
+        call move_alloc(tmp%JA, mats(slot)%JA)
+        call move_alloc(tmp%vA, mats(slot)%vA)
+    end subroutine take
+
+    subroutine ref_info(slot, m, n, nnz) bind(C, name='ref_info')
+        integer(c_int), value :: slot
+        integer(c_int), intent(out) :: m, n, nnz
+        m = mats(slot)%m
+        n = mats(slot)%n
+        nnz = mats(slot)%nnz
+    end subroutine ref_info
+
+    subroutine ref_copy_out(slot, nia, IA, JA, vA) bind(C, name='ref_copy_out')
+        integer(c_int), value :: slot, nia
+        integer(c_int), intent(out) :: IA(nia), JA(*)
+        real(c_double), intent(out) :: vA(*)
+        integer :: nnz
+        nnz = mats(slot)%nnz
+        IA = mats(slot)%IA
+        JA(1:nnz) = mats(slot)%JA
+        vA(1:nnz) = mats(slot)%vA
+    end subroutine ref_copy_out
+
+    subroutine ref_set(slot, m, n, nnz, IA, JA, vA) bind(C, name='ref_set')
+        integer(c_int), value :: slot, m, n, nnz
+        integer(c_int), intent(in) :: IA(m + 1), JA(nnz)
+        real(c_double), intent(in) :: vA(nnz)
+        if (allocated(mats(slot)%IA)) deallocate (mats(slot)%IA)
+        if (allocated(mats(slot)%JA)) deallocate (mats(slot)%JA)
+        if (allocated(mats(slot)%vA)) deallocate (mats(slot)%vA)
+        mats(slot)%m = m
+        mats(slot)%n = n
+        mats(slot)%nnz = nnz
+        allocate (mats(slot)%IA(0:m), mats(slot)%JA(nnz), mats(slot)%vA(nnz))
+        mats(slot)%IA(0:m) = IA
+        mats(slot)%JA = JA
+        mats(slot)%vA = vA
+    end subroutine ref_set
+
+    subroutine ref_matvec(slot, n, x, m, y) bind(C, name='ref_matvec')
+        integer(c_int), value :: slot, n, m
+        real(c_double), intent(in) :: x(n)
+        real(c_double), intent(out) :: y(m)
+        y = mats(slot)%matvecmul(x)
+    end subroutine ref_matvec
+
+    subroutine ref_transvec(slot, m, y, n, x) bind(C, name='ref_transvec')
+        integer(c_int), value :: slot, m, n
+        real(c_double), intent(in) :: y(m)
+        real(c_double), intent(out) :: x(n)
+        x = mats(slot)%transvecmul(y)
+    end subroutine ref_transvec
+
+    subroutine ref_rowvecmult(slot, k, n, v, f) bind(C, name='ref_rowvecmult')
+        integer(c_int), value :: slot, k, n
+        real(c_double), intent(in) :: v(n)
+        real(c_double), intent(out) :: f
+        f = mats(slot)%rowvecmult(k, v)
+    end subroutine ref_rowvecmult
+
+    subroutine ref_row_norms(slot, p, m, out) bind(C, name='ref_row_norms')
+        integer(c_int), value :: slot, p, m
+        real(c_double), intent(out) :: out(m)
+        out = mats(slot)%row_norms(p)
+    end subroutine ref_row_norms
+
+    subroutine ref_transpose(slot) bind(C, name='ref_transpose')
+        integer(c_int), value :: slot
+        call mats(slot)%transpose()
+    end subroutine ref_transpose
+
+    subroutine ref_scale(slot, c) bind(C, name='ref_scale')
+        integer(c_int), value :: slot
+        real(c_double), value :: c
+        real(kind=8) :: cc
+        cc = c
+        call mats(slot)%multiply_by_scalar(cc)
+    end subroutine ref_scale
+
+    subroutine ref_add_mult_row(slot, k, c, n, x, lwrb, lb, upb, ub) bind(C, name='ref_add_mult_row')
+        integer(c_int), value :: slot, k, n, lwrb, upb
+        real(c_double), value :: c, lb, ub
+        real(c_double), intent(inout) :: x(n)
+        real(kind=8) :: cc
+        cc = c
+        if (lwrb /= 0 .and. upb /= 0) then
+            call mats(slot)%add_scalar_mult_row(k, cc, x, lwrb=lb, upb=ub)
+        else if (lwrb /= 0) then
+            call mats(slot)%add_scalar_mult_row(k, cc, x, lwrb=lb)
+        else if (upb /= 0) then
+            call mats(slot)%add_scalar_mult_row(k, cc, x, upb=ub)
+        else
+            call mats(slot)%add_scalar_mult_row(k, cc, x)
+        end if
+    end subroutine ref_add_mult_row
+
+    subroutine fill_opts(options, upb, lwrb, ub, lb, relax, damp, rowprob)
+        type(opts), intent(out) :: options
+        integer(c_int), value :: upb, lwrb, rowprob
+        real(c_double), value :: ub, lb, relax, damp
+        call options%initialise()
+        if (upb /= 0) call options%set_ubound(ub)
+        if (lwrb /= 0) call options%set_lbound(lb)
+        options%relaxpar = relax
+        options%damp = damp
+        options%rowprobability = (rowprob /= 0)
+    end subroutine fill_opts
+
+    ! method 0 = cimmino, 1 = sart.  Note: SART re-sorts the slot's matrix in place, as the
+    ! reference does.
+    subroutine ref_sirt(slot, method, m, b, maxits, hasx0, n, x0, upb, lwrb, ub, lb, relax, xout) &
+        bind(C, name='ref_sirt')
+        integer(c_int), value :: slot, method, m, maxits, hasx0, n, upb, lwrb
+        real(c_double), value :: ub, lb, relax
+        real(c_double), intent(in) :: b(m), x0(n)
+        real(c_double), intent(out) :: xout(n)
+        type(opts) :: options
+        real(kind=8) :: xo(n)
+        call fill_opts(options, upb, lwrb, ub, lb, relax, 0d0, 0)
+        if (method == 0) then
+            if (hasx0 /= 0) then
+                call sirt(xo, 'cimmino', mats(slot), b, (/maxits/), x0=x0, options=options)
+            else
+                call sirt(xo, 'cimmino', mats(slot), b, (/maxits/), options=options)
+            end if
+        else
+            if (hasx0 /= 0) then
+                call sirt(xo, 'sart', mats(slot), b, (/maxits/), x0=x0, options=options)
+            else
+                call sirt(xo, 'sart', mats(slot), b, (/maxits/), options=options)
+            end if
+        end if
+        xout = xo
+    end subroutine ref_sirt
+
+    subroutine ref_gd(slot, m, b, maxiter, hasx0, n, x0, rho, tol, upb, lwrb, ub, lb, xout) &
+        bind(C, name='ref_gd')
+        integer(c_int), value :: slot, m, maxiter, hasx0, n, upb, lwrb
+        real(c_double), value :: rho, tol, ub, lb
+        real(c_double), intent(in) :: b(m), x0(n)
+        real(c_double), intent(out) :: xout(n)
+        type(opts) :: options
+        real(kind=8) :: xo(n), rr, tt
+        integer :: mi
+        call fill_opts(options, upb, lwrb, ub, lb, 1d0, 0d0, 0)
+        rr = rho
+        tt = tol
+        mi = maxiter
+        if (hasx0 /= 0) then
+            call gradient_descent(xo, mats(slot), b, max_iter=mi, x0=x0, rho_in=rr, tolerance=tt, options=options)
+        else
+            call gradient_descent(xo, mats(slot), b, max_iter=mi, rho_in=rr, tolerance=tt, options=options)
+        end if
+        xout = xo
+    end subroutine ref_gd
+
+    function ref_normest(slot, tol, maxit) bind(C, name='ref_normest') result(sigma)
+        integer(c_int), value :: slot, maxit
+        real(c_double), value :: tol
+        real(c_double) :: sigma
+        real(kind=8) :: tt
+        integer :: mi
+        tt = tol
+        mi = maxit
+        sigma = normest(mats(slot), tt, mi)
+    end function ref_normest
+
+    ! method 0 = kaczmarz, 1 = symkaczmarz, 2 = randkaczmarz.
+    subroutine ref_art(slot, method, m, b, nk, kin, hasx0, n, x0, hasorder, nord, order, &
+                       upb, lwrb, ub, lb, relax, damp, rowprob, xout) bind(C, name='ref_art')
+        integer(c_int), value :: slot, method, m, nk, hasx0, n, hasorder, nord, upb, lwrb, rowprob
+        real(c_double), value :: ub, lb, relax, damp
+        real(c_double), intent(in) :: b(m), x0(n)
+        integer(c_int), intent(in) :: kin(nk), order(*)
+        real(c_double), intent(out) :: xout(n, nk)
+        type(opts) :: options
+        character(len=12) :: meth
+        real(kind=8) :: xo(n, nk)
+        integer, allocatable :: ordv(:)
+        call fill_opts(options, upb, lwrb, ub, lb, relax, damp, rowprob)
+        select case (method)
+        case (0)
+            meth = 'kaczmarz'
+        case (1)
+            meth = 'symkaczmarz'
+        case default
+            meth = 'randkaczmarz'
+        end select
+        xo = 0d0
+        if (hasorder /= 0) then
+            allocate (ordv(nord))
+            ordv = order(1:nord)
+            if (hasx0 /= 0) then
+                call art(xo, trim(meth), mats(slot), b, kin, x0=x0, order=ordv, options=options)
+            else
+                call art(xo, trim(meth), mats(slot), b, kin, order=ordv, options=options)
+            end if
+        else
+            if (hasx0 /= 0) then
+                call art(xo, trim(meth), mats(slot), b, kin, x0=x0, options=options)
+            else
+                call art(xo, trim(meth), mats(slot), b, kin, options=options)
+            end if
+        end if
+        xout = xo
+    end subroutine ref_art
+
+    ! Seed the runtime generator deterministically from one integer.
+    subroutine ref_rng_seed(seed) bind(C, name='ref_rng_seed')
+        integer(c_int), value :: seed
+        integer :: k, i
+        integer, allocatable :: s(:)
+        call random_seed(size=k)
+        allocate (s(k))
+        do i = 1, k
+            s(i) = seed + 7919*i
+        end do
+        call random_seed(put=s)
+    end subroutine ref_rng_seed
+
+    subroutine ref_rng_draw(cnt, u) bind(C, name='ref_rng_draw')
+        integer(c_int), value :: cnt
+        real(c_double), intent(out) :: u(cnt)
+        integer :: i
+        real(kind=8) :: v
+        do i = 1, cnt
+            call random_number(v)
+            u(i) = v
+        end do
+    end subroutine ref_rng_draw
+
+    subroutine ref_shepplogan(nn, x) bind(C, name='ref_shepplogan')
+        integer(c_int), value :: nn
+        real(c_double), intent(out) :: x(nn*nn)
+        x = vectorise(shepplogan(nn))
+    end subroutine ref_shepplogan
+
+    subroutine ref_linspace(a, b, nn, x) bind(C, name='ref_linspace')
+        real(c_double), value :: a, b
+        integer(c_int), value :: nn
+        real(c_double), intent(out) :: x(nn)
+        real(kind=8) :: aa, bb
+        aa = a
+        bb = b
+        x = linspace(aa, bb, nn)
+    end subroutine ref_linspace
+
+    ! Fortran SUM / NORM2 / MIN / MAX probes used to pin the oracle's reductions.
+    function ref_sum(nn, v) bind(C, name='ref_sum') result(s)
+        integer(c_int), value :: nn
+        real(c_double), intent(in) :: v(nn)
+        real(c_double) :: s
+        s = sum(v)
+    end function ref_sum
+end module ref_harness

@@ -1,0 +1,190 @@
+"""Write the golden fixtures of tests/golden/ from the reference's own Fortran.
+
+Runs ONLY where the reference sources exist: it loads oracle/_ref/libfair_ref.so (built from
+them by `make -C oracle ref`) and calls the reference procedures (fanlineartomo, paralleltomo,
+matvecmul, transvecmul, row_norms, transpose, sirt, gradient_descent, normest, art) on this
+project's seeded inputs.  The reference runs single-threaded: its OpenMP A^T*y scatter
+(sparse_matrices.f90:382-392) races with more than one thread.
+
+Outputs (numpy .npz, inputs and expected outputs only):
+  sysgen_small.npz   full CSR matrices of small fan / parallel geometries
+  c1.npz             the N=64 plain-method problem: matrix hashes + sampled rows, b, and the
+                     SART / Cimmino / gradient-descent iterates
+  art_small.npz      Kaczmarz family on a small fan problem, with the uniform streams drawn
+  c2.npz             (--c2) the N=512 problem: matrix hashes + sampled rows, SART and Cimmino
+                     200-iteration iterates
+
+usage: python tests/golden/make_golden.py [--c2]
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+os.environ["OMP_NUM_THREADS"] = "1"
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from fortran_airtools_amd import configs as cf  # noqa: E402
+from oracle.oracle import Reference  # noqa: E402
+
+PI = np.pi
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+SMALL = [
+    # name, beam, n, p, nth, r, dw, sd, theta(rad)
+    ("fan_8_5", "fan", 8, 5, 6, 2.0, 3.1, 4.0, np.arange(6) * 60.0 / 180 * PI),
+    ("fan_8_6", "fan", 8, 6, 6, 2.0, 3.1, 4.0, np.arange(6) * 60.0 / 180 * PI),
+    ("fan_15_31", "fan", 15, 31, 12, 2.0, 3.1, 4.0, np.arange(12) * 30.0 / 180 * PI),
+    ("fan_16_21", "fan", 16, 21, 36, 2.0, 3.1, 4.0, np.arange(36) * 10.0 / 180 * PI),
+    ("fan_16_20", "fan", 16, 20, 36, 2.0, 3.1, 4.0, np.arange(36) * 10.0 / 180 * PI),
+    ("fan_32_41", "fan", 32, 41, 60, 2.0, 3.1, 4.0, np.arange(60) * 6.0 / 180 * PI),
+    ("fan_24_64_wide", "fan", 24, 64, 20, 1.5, 4.0, 2.5, np.arange(20) * 18.0 / 180 * PI),
+    ("par_8_12", "parallel", 8, 12, 4, 0, 0, 0, cf.linspace(0.0, PI, 4)),
+    ("par_15_20", "parallel", 15, 20, 10, 0, 0, 0, cf.linspace(0.0, PI, 10)),
+    ("par_16_23", "parallel", 16, 23, 30, 0, 0, 0, cf.linspace(0.0, PI, 30)),
+    ("par_16_24", "parallel", 16, 24, 30, 0, 0, 0, cf.linspace(0.0, PI, 30)),
+    ("par_32_48", "parallel", 32, 48, 40, 0, 0, 0, np.arange(40) * 4.5 / 180 * PI),
+]
+
+
+def build(ref, slot, beam, n, p, nth, r, dw, sd, th):
+    ref.build(slot, 0 if beam == "fan" else 1, n, p, nth, r, dw, sd, th)
+    return ref.get(slot)
+
+
+def sample_rows(IA, every):
+    rows = np.arange(0, len(IA) - 1, every, dtype=np.int64)
+    rows = rows[np.diff(IA)[rows] > 0]
+    return rows
+
+
+def write_small(ref):
+    out = {}
+    for name, beam, n, p, nth, r, dw, sd, th in SMALL:
+        IA, JA, vA = build(ref, 0, beam, n, p, nth, r, dw, sd, th)
+        out[f"{name}/geom"] = np.array([n, p, nth], np.int64)
+        out[f"{name}/rdwsd"] = np.array([r, dw, sd], np.float64)
+        out[f"{name}/theta"] = np.asarray(th, np.float64)
+        out[f"{name}/beam"] = np.array(beam)
+        out[f"{name}/IA"], out[f"{name}/JA"], out[f"{name}/vA"] = IA, JA, vA
+    np.savez_compressed(os.path.join(HERE, "sysgen_small.npz"), **out)
+
+
+def problem(ref, g: cf.Geometry, phantom, eta, seed):
+    th = g.theta()
+    IA, JA, vA = build(ref, 0, g.beam, g.n, g.p, g.nth, g.r, g.dw, g.sd, th)
+    xt = phantom(g.n)
+    b = cf.add_noise(ref.matvec(0, xt), eta, seed)
+    return IA, JA, vA, xt, b
+
+
+def matrix_summary(prefix, IA, JA, vA, every):
+    rows = sample_rows(IA, every)
+    d = {f"{prefix}IA_sha": np.array(sha(IA)), f"{prefix}JA_sha": np.array(sha(JA)),
+         f"{prefix}vA_sha": np.array(sha(vA)), f"{prefix}nnz": np.array(len(JA)),
+         f"{prefix}sample_rows": rows}
+    d[f"{prefix}sample_IA"] = np.stack([IA[rows], IA[rows + 1]], 1)
+    sel = np.concatenate([np.arange(IA[k], IA[k + 1]) for k in rows])
+    d[f"{prefix}sample_JA"], d[f"{prefix}sample_vA"] = JA[sel], vA[sel]
+    return d
+
+
+def write_c1(ref):
+    g = cf.C1
+    IA, JA, vA, xt, b = problem(ref, g, cf.shepplogan, 0.01, 1)
+    n = g.ncols
+    d = matrix_summary("", IA, JA, vA, 97)
+    d.update(IA=IA, xt=xt, b=b)
+    rng = np.random.default_rng(11)
+    x = rng.random(n)
+    y = rng.standard_normal(g.m)
+    d["probe_x"], d["probe_Ax"] = x, ref.matvec(0, x)
+    d["probe_y"], d["probe_ATy"] = y, ref.transvec(0, y)
+    d["rownorm1"], d["rownorm2"] = ref.row_norms(0, 1), ref.row_norms(0, 2)
+    for k in (1, 10, 100):
+        ref.set(0, IA, JA, vA, n)
+        d[f"sart_{k}"] = ref.sirt(0, "sart", b, k, lb=0.0, ub=1.0, relax=1.0)
+    ref.set(0, IA, JA, vA, n)
+    d["sart_norelax_box_none_20"] = ref.sirt(0, "sart", b, 20, relax=1.5)
+    for k in (1, 100):
+        ref.set(0, IA, JA, vA, n)
+        d[f"cimmino_{k}"] = ref.sirt(0, "cimmino", b, k, lb=0.0, ub=1.0, relax=1.0)
+    ref.set(0, IA, JA, vA, n)
+    ref.rng_seed(21)
+    d["normest_sigma"] = np.array(ref.normest(0))
+    ref.rng_seed(21)
+    d["normest_v0"] = ref.rng_draw(n)
+    ref.rng_seed(21)
+    d["gd_100"] = ref.gd(0, b, 100, lb=0.0, ub=1.0)
+    ref.rng_seed(21)
+    d["gd_tol_1e-3"] = ref.gd(0, b, 10000, tol=1e-3)
+    np.savez_compressed(os.path.join(HERE, "c1.npz"), **d)
+
+
+def write_art(ref):
+    g = cf.Geometry(n=16, p=21, nth=30, deg_step=12.0)
+    IA, JA, vA, xt, b = problem(ref, g, cf.shepplogan, 0.01, 5)
+    n, m = g.ncols, g.m
+    d = {"IA": IA, "JA": JA, "vA": vA, "b": b, "xt": xt}
+    cases = [("kaczmarz", [1, 3, 5], False, {}), ("kaczmarz", [4], False, {"lb": 0.0, "ub": 1.0}),
+             ("symkaczmarz", [2, 6], False, {"lb": 0.0, "ub": 1.0}),
+             ("randkaczmarz", [2, 4], False, {"lb": 0.0, "ub": 1.0}),
+             ("randkaczmarz", [3], True, {"lb": 0.0, "ub": 1.0}),
+             ("randkaczmarz", [2], True, {"ub": 1.0, "relax": 0.7, "damp": 0.01}),
+             ("kaczmarz", [2], False, {"ub": 1.0, "relax": 1.4, "damp": 0.05})]
+    for i, (meth, kin, rp, kw) in enumerate(cases):
+        ref.rng_seed(100 + i)
+        X = ref.art(0, meth, b, kin, rowprob=rp, **kw)
+        ref.rng_seed(100 + i)
+        u = ref.rng_draw(2 * m + max(kin) * 3 * m)
+        d[f"case{i}/method"] = np.array(meth)
+        d[f"case{i}/kin"] = np.array(kin, np.int32)
+        d[f"case{i}/rowprob"] = np.array(rp)
+        d[f"case{i}/lb"] = np.array(kw.get("lb", np.nan))
+        d[f"case{i}/ub"] = np.array(kw.get("ub", np.nan))
+        d[f"case{i}/relax"] = np.array(kw.get("relax", 1.0))
+        d[f"case{i}/damp"] = np.array(kw.get("damp", 0.0))
+        d[f"case{i}/u"] = u
+        d[f"case{i}/X"] = X
+    order = (np.random.default_rng(3).permutation(m) + 1).astype(np.int32)
+    d["order"] = order
+    d["order_X"] = ref.art(0, "kaczmarz", b, [3], order=order, lb=0.0, ub=1.0)
+    np.savez_compressed(os.path.join(HERE, "art_small.npz"), **d)
+
+
+def write_c2(ref):
+    g = cf.C2
+    IA, JA, vA, xt, b = problem(ref, g, cf.grains, 0.005, 2)
+    n = g.ncols
+    d = matrix_summary("", IA, JA, vA, 1009)
+    d["b"] = b
+    ref.set(0, IA, JA, vA, n)
+    d["sart_200"] = ref.sirt(0, "sart", b, 200, lb=0.0, ub=1.0, relax=1.0)
+    ref.set(0, IA, JA, vA, n)
+    d["cimmino_200"] = ref.sirt(0, "cimmino", b, 200, lb=0.0, ub=1.0, relax=1.0)
+    np.savez_compressed(os.path.join(HERE, "c2.npz"), **d)
+
+
+def main():
+    if not Reference.available():
+        sys.exit("oracle/_ref/libfair_ref.so is not built (make -C oracle ref)")
+    ref = Reference()
+    if "--c2" in sys.argv:
+        write_c2(ref)
+        return
+    write_small(ref)
+    write_c1(ref)
+    write_art(ref)
+
+
+if __name__ == "__main__":
+    main()
