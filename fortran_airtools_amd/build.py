@@ -81,6 +81,20 @@ def build_fortran(verbose: bool = False) -> str | None:
     subprocess.run(["ar", "rcs", lib, *objs], check=True)
     if verbose:
         print("built", lib)
+    # example callers (reference-style programs linking the drop-in modules)
+    bindir = os.path.join(PKG, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    exdir = os.path.join(FORTRAN, "examples")
+    for f in sorted(os.listdir(exdir)) if os.path.isdir(exdir) else []:
+        if not f.endswith(".f90"):
+            continue
+        exe = os.path.join(bindir, f[:-4])
+        r = subprocess.run([FLANG, "-O2", "-I", mod, os.path.join(exdir, f), lib, "-L", PKG, "-lfair_hip",
+                            "-Wl,-rpath,$ORIGIN/..", "-o", exe], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed for {f}:\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print("built", exe)
     return lib
 
 

@@ -170,8 +170,14 @@ PHANTOMS = {"shepplogan": shepplogan, "grains": grains, "bubbles": bubbles, "smo
             "3phases": threephases, "blobs": blobs}
 
 
+def norm2(v: np.ndarray) -> float:
+    """Euclidean norm with numpy's fixed pairwise summation (independent of BLAS threading)."""
+    v = np.asarray(v, np.float64)
+    return float(np.sqrt(np.sum(v * v)))
+
+
 def add_noise(b: np.ndarray, eta: float, seed: int) -> np.ndarray:
     """b + e with e = eta*||b||*g/||g||, g standard normal (the rescaling of
     system_generation.f90:742-743; the draws are this project's seeded generator)."""
     g = np.random.default_rng(seed).standard_normal(len(b))
-    return b + eta * np.linalg.norm(b) * g / np.linalg.norm(g)
+    return b + eta * norm2(b) * g / norm2(g)

@@ -90,6 +90,9 @@ struct DBuf {
 // ---- device CSR ----------------------------------------------------------------------------
 // Row offsets IA (m+1, int32, zero-based), column indices JA (int32, ZERO-based on the device;
 // the host boundary is one-based), values vA (double).
+// Spare entries after JA / vA: the SpMV's aligned pair loads may touch one entry past a row.
+constexpr int CSR_PAD = 512;
+
 struct Csr {
     int32_t m = 0, n = 0;
     int64_t nnz = 0;
@@ -123,6 +126,7 @@ namespace fair {
 
 // ---- kernels shared across translation units (defined in sparse.hip) -----------------------
 void build_row_list(Csr &A);
+void alloc_entries(Csr &A, int64_t nnz);          // JA / vA with CSR_PAD zeroed spare entries
 void build_transpose(const Csr &A, Csr &T);      // T = CSR(A^T), rows sorted by column
 void sort_rows(Csr &A);                           // sort each row's entries by column
 void ensure_T(fair_csr *h);

@@ -399,8 +399,7 @@ void trace(const fair_geom &geo, const std::vector<double> &ang, const std::vect
     A.n = geo.n * geo.n;
     A.IA.alloc(m + 1);
     A.nnz = exclusive_scan_i32(counts.p, m, A.IA.p);
-    A.JA.alloc(A.nnz > 0 ? A.nnz : 1);
-    A.vA.alloc(A.nnz > 0 ? A.nnz : 1);
+    alloc_entries(A, A.nnz);
     if (m > 0)
         hipLaunchKernelGGL(k_fill<BEAM>, dim3(grid_for(m, 256)), dim3(256), 0, stream(), g, m,
                            A.IA.p, irregular.p, A.JA.p, A.vA.p, err.p);

@@ -254,6 +254,13 @@ void fill(double *y, int64_t n, double v) {
     if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n, 256)), dim3(256), 0, stream(), y, n, v);
 }
 
+void alloc_entries(Csr &A, int64_t nnz) {
+    A.JA.alloc((size_t)nnz + CSR_PAD);
+    A.vA.alloc((size_t)nnz + CSR_PAD);
+    FAIR_HIP(hipMemsetAsync(A.JA.p + nnz, 0, sizeof(int32_t) * CSR_PAD, stream()));
+    FAIR_HIP(hipMemsetAsync(A.vA.p + nnz, 0, sizeof(double) * CSR_PAD, stream()));
+}
+
 void build_row_list(Csr &A) {
     int32_t m = A.m;
     DBuf<int32_t> flag(m > 0 ? m : 1), pos((size_t)m + 1), mx(1);
@@ -326,8 +333,7 @@ void build_transpose(const Csr &A, Csr &T) {
                            A.n, cnt.p);
     T.IA.alloc((size_t)T.m + 1);
     T.nnz = exclusive_scan_i32(cnt.p, T.m, T.IA.p);
-    T.JA.alloc(T.nnz > 0 ? T.nnz : 1);
-    T.vA.alloc(T.nnz > 0 ? T.nnz : 1);
+    alloc_entries(T, T.nnz);
     if (T.m > 0) FAIR_HIP(hipMemcpyAsync(cnt.p, T.IA.p, sizeof(int32_t) * T.m, hipMemcpyDeviceToDevice, stream()));
     if (A.n_rows_nz > 0)
         hipLaunchKernelGGL(k_transpose_scatter, dim3(grid_for((int64_t)A.n_rows_nz * WAVE, 256)), dim3(256), 0,
@@ -426,8 +432,7 @@ extern "C" int fair_csr_create(int32_t m, int32_t n, int64_t nnz, const int32_t 
             A.nnz = nnz;
             A.IA.alloc((size_t)m + 1);
             A.IA.upload(IA, (size_t)m + 1);
-            A.JA.alloc(nnz > 0 ? nnz : 1);
-            A.vA.alloc(nnz > 0 ? nnz : 1);
+            alloc_entries(A, nnz);
             if (nnz > 0) {
                 DBuf<int32_t> tmp(nnz);
                 tmp.upload(JA, nnz);

@@ -1,0 +1,129 @@
+! system_generation.f90 -- drop-in for the reference module of the same name
+! (system_generation.f90 of FAIR tools): the geometry record, linspace, the fan / parallel
+! beam system-matrix builders (run on the gfx950 device, bit-identical to the reference's
+! serial build), the text data reader and the white-noise generator.
+module system_generation
+    use iso_c_binding
+    use fair_capi
+    use sparse_matrices
+    implicit none
+    real(kind=8), parameter :: pi = 4d0*atan(1d0)
+
+    ! type(data), system_generation.f90:6-12
+    type data
+        integer :: n, p, nth
+        real(kind=8) :: r, dw, sd
+        real(kind=8), allocatable :: T(:), theta(:)
+    contains
+        procedure :: read => read_data
+    end type data
+
+contains
+
+    ! system_generation.f90:17-35 (x(i) = c*i + d), evaluated by the library
+    function linspace(a, b, n) result(x)
+        real(kind=8), intent(in) :: a, b
+        integer, intent(in) :: n
+        real(kind=8) :: x(n)
+        call fair_check(c_fair_linspace(real(a, c_double), real(b, c_double), int(n, c_int32_t), x))
+    end function linspace
+
+    function geom_of(dat) result(g)
+        class(data), intent(in) :: dat
+        type(fair_geom_c) :: g
+        if (.not. allocated(dat%theta)) stop "Angle vector is not allocated."
+        if (size(dat%theta) /= dat%nth) stop "Angle vector has the wrong length, nth is incorrect."
+        g%n = dat%n
+        g%p = dat%p
+        g%nth = dat%nth
+        g%r = dat%r
+        g%dw = dat%dw
+        g%sd = dat%sd
+    end function geom_of
+
+    ! fanlineartomo, system_generation.f90:79-307
+    function fanlineartomo(dat) result(Amat)
+        class(data), intent(in) :: dat
+        type(csr_matrix) :: Amat
+        type(fair_geom_c) :: g
+        real(c_double), allocatable :: th(:)
+        call fair_ensure_init()
+        g = geom_of(dat)
+        th = dat%theta
+        call fair_check(c_fair_fanlineartomo(g, th, Amat%dev))
+        call Amat%sync_host()
+    end function fanlineartomo
+
+    ! paralleltomo, system_generation.f90:309-492
+    function paralleltomo(dat) result(Amat)
+        class(data), intent(in) :: dat
+        type(csr_matrix) :: Amat
+        type(fair_geom_c) :: g
+        real(c_double), allocatable :: th(:)
+        call fair_ensure_init()
+        g = geom_of(dat)
+        th = dat%theta
+        call fair_check(c_fair_paralleltomo(g, th, Amat%dev))
+        call Amat%sync_host()
+    end function paralleltomo
+
+    ! read_data, system_generation.f90:494-539: "label value" lines for n, p, nth, r, dw, sd,
+    ! then "label count" + angles in degrees, then "label count" + transmission data.
+    subroutine read_data(dat, inputfile, verbose)
+        class(data), intent(out) :: dat
+        character(30), intent(in) :: inputfile
+        logical, optional :: verbose
+        character(30) :: label
+        integer :: cnt, i, u
+        logical :: verb
+        verb = .false.
+        if (present(verbose)) verb = verbose
+        if (verb) write (*, *) "Opening file: ", trim(inputfile)
+        open (newunit=u, file=trim(inputfile), action='read')
+        read (u, *) label, dat%n
+        read (u, *) label, dat%p
+        read (u, *) label, dat%nth
+        read (u, *) label, dat%r
+        read (u, *) label, dat%dw
+        read (u, *) label, dat%sd
+        read (u, *) label, cnt
+        if (cnt /= dat%nth) stop "Angle vector has the wrong length, nth is incorrect."
+        allocate (dat%theta(cnt))
+        do i = 1, cnt
+            read (u, *) dat%theta(i)
+        end do
+        dat%theta = dat%theta/180d0*pi
+        read (u, *) label, cnt
+        if (cnt /= dat%nth*dat%p) stop "Data has the wrong size, or p and nth are incorrect."
+        allocate (dat%T(cnt))
+        do i = 1, cnt
+            read (u, *) dat%T(i)
+        end do
+        close (u)
+        if (verb) write (*, *) "Closing file: ", trim(inputfile)
+    end subroutine read_data
+
+    ! add_white_noise, system_generation.f90:709-745: per sample a uniform draw u, a Newton
+    ! solve of Phi(x) = u from x = 0 with the reference's step f/df, df = sqrt(2/pi)exp(-x^2/2),
+    ! then e = eta*||b||*e/||e|| and bt = b + e.
+    subroutine add_white_noise(bt, e, b, eta)
+        real(kind=8), intent(in) :: b(:), eta
+        real(kind=8), dimension(size(b)) :: bt, e
+        real(kind=8) :: u, x, f, df
+        integer :: i, k
+        do k = 1, size(b)
+            call random_number(u)
+            x = 0d0
+            do i = 1, 100
+                f = .5d0*(1 + erf(x/sqrt(2d0))) - u
+                df = sqrt(2d0/pi)*exp(-.5d0*x**2)
+                if (abs(f/df) < 1d-14) exit
+                if (i == 100) stop "Newton method did not converge."
+                x = x - f/df
+            end do
+            e(k) = x
+        end do
+        e = eta*norm2(b)*e/norm2(e)
+        bt = b + e
+    end subroutine add_white_noise
+end module system_generation

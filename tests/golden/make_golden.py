@@ -11,8 +11,8 @@ Outputs (numpy .npz, inputs and expected outputs only):
   c1.npz             the N=64 plain-method problem: matrix hashes + sampled rows, b, and the
                      SART / Cimmino / gradient-descent iterates
   art_small.npz      Kaczmarz family on a small fan problem, with the uniform streams drawn
-  c2.npz             (--c2) the N=512 problem: matrix hashes + sampled rows, SART and Cimmino
-                     200-iteration iterates
+  c2.npz             (--c2) the N=512 problem: matrix hashes + sampled rows, the hash of b, SART
+                     and Cimmino 200-iteration iterates
 
 usage: python tests/golden/make_golden.py [--c2]
 """
@@ -165,8 +165,8 @@ def write_c2(ref):
     g = cf.C2
     IA, JA, vA, xt, b = problem(ref, g, cf.grains, 0.005, 2)
     n = g.ncols
-    d = matrix_summary("", IA, JA, vA, 1009)
-    d["b"] = b
+    d = matrix_summary("", IA, JA, vA, 10007)
+    d["b_sha"] = np.array(sha(b))      # b is rebuilt by the tests (oracle matvec + seeded noise)
     ref.set(0, IA, JA, vA, n)
     d["sart_200"] = ref.sirt(0, "sart", b, 200, lb=0.0, ub=1.0, relax=1.0)
     ref.set(0, IA, JA, vA, n)

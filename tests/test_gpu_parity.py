@@ -223,3 +223,35 @@ def test_size_errors():
         am.art("foo", A, g["b"], [3])
     with pytest.raises(FairError, match="Not a recognised SIRT"):
         am.sirt("foo", A, g["b"], [3])
+
+
+def c2_matrix():
+    from fortran_airtools_amd import configs as cf
+    g = cf.C2
+    return build("fan", g.n, g.p, g.nth, g.r, g.dw, g.sd, g.theta())
+
+
+def test_build_c2_bitexact():
+    gold = load_golden("c2.npz")
+    A = c2_matrix()
+    assert A.nnz == int(gold["nnz"])
+    IA, JA, vA = A.components()
+    assert sha(IA) == str(gold["IA_sha"])
+    assert sha(JA) == str(gold["JA_sha"])
+    assert sha(vA) == str(gold["vA_sha"])
+
+
+@pytest.mark.parametrize("method", ["sart", "cimmino"])
+def test_sirt_c2_200(method, oracle):
+    from fortran_airtools_amd import configs as cf
+    _, _, am = fair()
+    gold = load_golden("c2.npz")
+    A = c2_matrix()
+    IA, JA, vA = A.components()
+    assert sha(JA) == str(gold["JA_sha"]) and sha(vA) == str(gold["vA_sha"])
+    # b = A x_true + noise with the reference's summation order (the oracle's matvec)
+    b = cf.add_noise(oracle.matvec(IA, JA, vA, cf.grains(cf.C2.n, 200, 2)), 0.005, 2)
+    assert sha(b) == str(gold["b_sha"])
+    o = am.Opts().set_lbound(0.0).set_ubound(1.0)
+    x = am.sirt(method, A, b, [200], options=o)
+    assert_close_1e12(x, gold[f"{method}_200"])
