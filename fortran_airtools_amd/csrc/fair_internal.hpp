@@ -93,6 +93,21 @@ struct DBuf {
 // Spare entries after JA / vA: the SpMV's aligned pair loads may touch one entry past a row.
 constexpr int CSR_PAD = 512;
 
+// Layout of the LDS-staged SpMV (blocked.hip): non-empty rows grouped into blocks whose
+// column footprint (per bucket of W consecutive columns, the [min,max] range touched) is
+// loaded into LDS with coalesced segment loads; entries then address LDS through a 16-bit
+// local index instead of gathering from global memory.
+struct Blocked {
+    bool ok = false;
+    int32_t nblocks = 0;
+    int32_t lds_elems = 0;      // footprint capacity per block (entries; the last slot holds 0)
+    int32_t n_direct = 0;       // blocks whose footprint does not fit (global gathers)
+    DBuf<int32_t> binfo;        // int4 per block {row_begin, row_end, seg_begin, seg_end}
+    DBuf<int32_t> rows;         // int4 row descriptors {row, start, end, 0}, grouped by block
+    DBuf<int32_t> segs;         // int4 {global column start, length, LDS offset, 0}
+    DBuf<uint16_t> loc;         // per entry: LDS slot of its column (parallel to JA)
+};
+
 struct Csr {
     int32_t m = 0, n = 0;
     int64_t nnz = 0;
@@ -103,6 +118,7 @@ struct Csr {
     DBuf<int32_t> rows_empty;  // ids of the empty rows
     int32_t n_rows_empty = 0;
     int32_t max_row = 0;     // longest row
+    Blocked blk;             // LDS-staged layout (built for ray-traced matrices)
 };
 
 }  // namespace fair
@@ -111,7 +127,12 @@ struct fair_csr {
     fair::Csr A;        // the matrix as the caller sees it
     fair::Csr T;        // CSR of A^T, built on demand
     bool have_T = false;
+    // The caller's rows are to be seen sorted by column (the reference's double transpose);
+    // the device keeps its own entry order and sorts on download.
     bool rows_sorted = false;
+    // Geometry of a ray-traced matrix (image side, rays per angle, angles); 0 when unknown.
+    int32_t g_n = 0, g_p = 0, g_nth = 0;
+    bool transposed = false;   // fair_transpose swapped the roles of A and T
     // SIRT weights cached across calls (index: FAIR_CIMMINO / FAIR_SART) and loop workspace;
     // dropped whenever the matrix changes.
     fair::DBuf<double> M[2], D;
@@ -129,6 +150,12 @@ void build_row_list(Csr &A);
 void alloc_entries(Csr &A, int64_t nnz);          // JA / vA with CSR_PAD zeroed spare entries
 void build_transpose(const Csr &A, Csr &T);      // T = CSR(A^T), rows sorted by column
 void sort_rows(Csr &A);                           // sort each row's entries by column
+// Blocked layouts: rays grouped RB at a time within one angle (columns bucketed by image
+// column), or pixels grouped into TX x TY tiles (columns bucketed by angle).
+void build_blocked_rays(Csr &A, int32_t img_n, int32_t p, int32_t RB, int32_t lds_elems);
+void build_blocked_tiles(Csr &T, int32_t img_n, int32_t p, int32_t nth, int32_t TX, int32_t TY,
+                         int32_t lds_elems);
+void ensure_blocked(fair_csr *h);
 void ensure_T(fair_csr *h);
 void invalidate_T(fair_csr *h);
 

@@ -277,10 +277,7 @@ void sirt_setup(fair_csr *h, int method) {
             hipLaunchKernelGGL(k_cimmino_weights, dim3(grid_for(m, 256)), dim3(256), 0, stream(), h->M[method].p, m);
         } else {
             row_norms(A, 1, h->M[method].p);          // AIRMs.f90:1076
-            if (!h->rows_sorted) {                    // :1077-1079
-                sort_rows(A);
-                h->rows_sorted = true;
-            }
+            h->rows_sorted = true;                    // :1077-1079 (downloads show sorted rows)
             ensure_T(h);
             h->D.alloc(n > 0 ? n : 1);
             row_norms(h->T, 1, h->D.p);               // column sums
@@ -289,6 +286,7 @@ void sirt_setup(fair_csr *h, int method) {
         h->w_ok[method] = true;
     }
     ensure_T(h);
+    ensure_blocked(h);
     if (h->ws_r.n < (size_t)(m > 0 ? m : 1)) h->ws_r.alloc(m > 0 ? m : 1);
     if (h->ws_x.n < (size_t)(n > 0 ? n : 1)) h->ws_x.alloc(n > 0 ? n : 1);
 }
@@ -370,6 +368,7 @@ extern "C" int fair_normest(fair_csr *h, const double *v0, double tol, int32_t m
         Csr &A = h->A;
         if (A.n_rows_nz < 0) build_row_list(A);
         ensure_T(h);
+        ensure_blocked(h);
         const int32_t m = A.m, n = A.n;
         DBuf<double> v(n > 0 ? n : 1), vn(n > 0 ? n : 1), w(m > 0 ? m : 1), part(RED_BLOCKS), out(1);
         v.upload(v0, n);
@@ -400,6 +399,7 @@ extern "C" int fair_gradient_descent(fair_csr *h, const double *b, int32_t maxit
         Csr &A = h->A;
         if (A.n_rows_nz < 0) build_row_list(A);
         ensure_T(h);
+        ensure_blocked(h);
         const Csr &T = h->T;
         const int32_t m = A.m, n = A.n;
         DBuf<double> db(m > 0 ? m : 1), r(m > 0 ? m : 1), x0b(n > 0 ? n : 1), x1b(n > 0 ? n : 1),

@@ -474,6 +474,9 @@ extern "C" int fair_fanlineartomo(const fair_geom *g, const double *theta, fair_
             delete h;
             throw;
         }
+        h->g_n = n;
+        h->g_p = p;
+        h->g_nth = nth;
         *out = h;
     });
 }
@@ -506,6 +509,9 @@ extern "C" int fair_paralleltomo(const fair_geom *g, const double *theta, fair_c
             delete h;
             throw;
         }
+        h->g_n = g->n;
+        h->g_p = p;
+        h->g_nth = nth;
         *out = h;
     });
 }

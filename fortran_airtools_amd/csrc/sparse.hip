@@ -285,38 +285,41 @@ void build_row_list(Csr &A) {
     A.max_row = hmx;
 }
 
-static void segsort_rows(Csr &A) {
-    if (A.n_rows_nz < 0) build_row_list(A);
-    if (A.n_rows_nz == 0 || A.max_row < 2) return;
-    const int4 *rows = (const int4 *)A.rows_nz.p;
+// Sort the entries of every listed row by column index (values follow).
+static void segsort_arrays(const int4 *rows, int32_t nrows, int32_t max_row, int32_t *JA, double *vA) {
+    if (nrows <= 0 || max_row < 2) return;
     int cap = 1;
     while (cap * 2 * (int)(sizeof(double) + sizeof(int32_t)) <= lds_per_block() && cap < 8192) cap *= 2;
     size_t shm = (size_t)cap * (sizeof(double) + sizeof(int32_t));
-    hipLaunchKernelGGL(k_segsort_lds, dim3(A.n_rows_nz), dim3(256), shm, stream(), rows,
-                       A.n_rows_nz, A.JA.p, A.vA.p, cap);
+    hipLaunchKernelGGL(k_segsort_lds, dim3(nrows), dim3(256), shm, stream(), rows, nrows, JA, vA, cap);
     launch_check("segmented sort");
-    if (A.max_row > cap) {
-        DBuf<int32_t> ids(A.n_rows_nz), nl(1);
+    if (max_row > cap) {
+        DBuf<int32_t> ids(nrows), nl(1);
         FAIR_HIP(hipMemsetAsync(nl.p, 0, sizeof(int32_t), stream()));
-        hipLaunchKernelGGL(k_long_rows, dim3(grid_for(A.n_rows_nz, 256)), dim3(256), 0, stream(), rows,
-                           A.n_rows_nz, cap, ids.p, nl.p);
+        hipLaunchKernelGGL(k_long_rows, dim3(grid_for(nrows, 256)), dim3(256), 0, stream(), rows, nrows, cap,
+                           ids.p, nl.p);
         int32_t nlong = 0;
         nl.download(&nlong, 1);
         FAIR_HIP(hipStreamSynchronize(stream()));
         int64_t P = 1;
-        while (P < A.max_row) P <<= 1;
+        while (P < max_row) P <<= 1;
         // bound the scratch: process long rows in batches
         const int32_t batch = (int32_t)std::max<int64_t>(1, std::min<int64_t>(nlong, (int64_t)(1 << 28) / P));
         DBuf<int32_t> ks((size_t)batch * P);
         DBuf<double> vs((size_t)batch * P);
         for (int32_t b0 = 0; b0 < nlong; b0 += batch) {
             int32_t nb = std::min(batch, nlong - b0);
-            hipLaunchKernelGGL(k_segsort_global, dim3(nb), dim3(256), 0, stream(), rows, ids.p + b0,
-                               nb, A.JA.p, A.vA.p, ks.p, vs.p, P);
+            hipLaunchKernelGGL(k_segsort_global, dim3(nb), dim3(256), 0, stream(), rows, ids.p + b0, nb, JA, vA,
+                               ks.p, vs.p, P);
         }
         launch_check("segmented sort (long rows)");
         FAIR_HIP(hipStreamSynchronize(stream()));
     }
+}
+
+static void segsort_rows(Csr &A) {
+    if (A.n_rows_nz < 0) build_row_list(A);
+    segsort_arrays((const int4 *)A.rows_nz.p, A.n_rows_nz, A.max_row, A.JA.p, A.vA.p);
 }
 
 void sort_rows(Csr &A) { segsort_rows(A); }
@@ -464,14 +467,22 @@ extern "C" int fair_csr_download(const fair_csr *h, int32_t *IA, int32_t *JA, do
         check_handle(h);
         const Csr &A = h->A;
         if (IA) A.IA.download(IA, (size_t)A.m + 1);
-        if (JA && A.nnz > 0) {
-            DBuf<int32_t> tmp(A.nnz);
+        if ((JA || vA) && A.nnz > 0) {
+            // the device keeps its own entry order; the caller sees the reference's
+            DBuf<int32_t> tj(A.nnz);
+            DBuf<double> tv;
+            const double *vsrc = A.vA.p;
             hipLaunchKernelGGL(k_add_i32, dim3(grid_for(A.nnz, 256)), dim3(256), 0, stream(), A.JA.p,
-                               tmp.p, A.nnz, 1);
-            tmp.download(JA, A.nnz);
-            FAIR_HIP(hipStreamSynchronize(stream()));
+                               tj.p, A.nnz, 1);
+            if (h->rows_sorted && A.max_row >= 2) {
+                tv.alloc(A.nnz);
+                FAIR_HIP(hipMemcpyAsync(tv.p, A.vA.p, sizeof(double) * A.nnz, hipMemcpyDeviceToDevice, stream()));
+                segsort_arrays((const int4 *)A.rows_nz.p, A.n_rows_nz, A.max_row, tj.p, tv.p);
+                vsrc = tv.p;
+            }
+            if (JA) tj.download(JA, A.nnz);
+            if (vA) FAIR_HIP(hipMemcpyAsync(vA, vsrc, sizeof(double) * A.nnz, hipMemcpyDeviceToHost, stream()));
         }
-        if (vA) A.vA.download(vA, A.nnz);
         FAIR_HIP(hipStreamSynchronize(stream()));
     });
 }
@@ -498,6 +509,7 @@ extern "C" int fair_csr_free(fair_csr *h) {
 extern "C" int fair_matvecmul_dev(fair_csr *h, const double *d_v, double *d_w) {
     return guard([&] {
         check_handle(h);
+        ensure_blocked(h);
         spmv(h->A, d_v, d_w);
     });
 }
@@ -505,6 +517,7 @@ extern "C" int fair_matvecmul_dev(fair_csr *h, const double *d_v, double *d_w) {
 extern "C" int fair_spmv_rows_dev(fair_csr *h, const double *d_v, double *d_w) {
     return guard([&] {
         check_handle(h);
+        ensure_blocked(h);
         spmv_launch(h->A, d_v, EpiStore<double>{d_w});
     });
 }
@@ -512,6 +525,7 @@ extern "C" int fair_spmv_rows_dev(fair_csr *h, const double *d_v, double *d_w) {
 extern "C" int fair_transvecmul_dev(fair_csr *h, const double *d_v, double *d_w) {
     return guard([&] {
         check_handle(h);
+        ensure_blocked(h);
         ensure_T(h);
         spmv(h->T, d_v, d_w);
     });
@@ -522,6 +536,7 @@ extern "C" int fair_matvecmul(fair_csr *h, const double *v, int32_t nv, double *
         check_handle(h);
         FAIR_CHECK(nv == h->A.n, FAIR_ERR_ARG, "Vector input size incorrect.");
         FAIR_CHECK(nw == h->A.m, FAIR_ERR_ARG, "Output size incorrect.");
+        ensure_blocked(h);
         DBuf<double> dv(nv > 0 ? nv : 1), dw(nw > 0 ? nw : 1);
         copy_in(dv.p, v, nv);
         spmv(h->A, dv.p, dw.p);
@@ -534,6 +549,7 @@ extern "C" int fair_transvecmul(fair_csr *h, const double *v, int32_t nv, double
         check_handle(h);
         FAIR_CHECK(nv == h->A.m, FAIR_ERR_ARG, "Vector input size incorrect.");
         FAIR_CHECK(nw == h->A.n, FAIR_ERR_ARG, "Output size incorrect.");
+        ensure_blocked(h);
         ensure_T(h);
         DBuf<double> dv(nv > 0 ? nv : 1), dw(nw > 0 ? nw : 1);
         copy_in(dv.p, v, nv);
@@ -583,14 +599,15 @@ extern "C" int fair_transpose(fair_csr *h) {
     return guard([&] {
         check_handle(h);
         ensure_T(h);
-        // the transposed matrix becomes the caller's matrix; its transpose is the old A with
-        // rows sorted by column (the reference's double transpose leaves that order)
+        // the transposed matrix becomes the caller's matrix (its rows are in counting-sort
+        // order); transposing back shows the old A with rows sorted by column, which the
+        // reference's double transpose leaves -- downloads sort, the device order stays
         Csr old = std::move(h->A);
         h->A = std::move(h->T);
-        sort_rows(old);
         h->T = std::move(old);
         h->have_T = true;
         h->rows_sorted = true;
+        h->transposed = !h->transposed;
         h->drop_cache();
     });
 }

@@ -1,13 +1,21 @@
-// spmv.hpp -- the CSR row-per-wavefront SpMV kernel with fused epilogues.
+// spmv.hpp -- CSR SpMV kernels with fused epilogues.
 //
-// A 64-lane wavefront owns SPMV_RPW consecutive non-empty rows of the compacted
-// {row,start,end} list (one descriptor load per wave).  Per row, every lane requests
-// SPMV_UNROLL aligned pairs of (value, index) with non-temporal loads -- 512 entries, so the
-// whole row is in flight before the first gather and the streamed matrix does not evict the
-// gathered vector from the XCD's L2 -- then gathers x[JA] (2 MiB at n = 512^2 in FP64, L2/MALL
-// resident), accumulates in two chains and reduces across the wavefront.  Column indices outside [0, ncols) contribute nothing (the reference
-// can emit such indices, see raytrace.hip).  Lane 0 hands the row result to an epilogue functor, which is where
-// SART / Cimmino / gradient-descent steps fuse their vector work.
+// Two kernels share the row walk: a 64-lane wavefront owns a row, every lane requests
+// SPMV_UNROLL aligned pairs of (value, column) with non-temporal loads (512 entries: a whole
+// average fan-beam row at N = 512 is in flight before the first use), accumulates in two chains
+// and reduces across the wavefront; lane 0 hands the row result to an epilogue functor, which
+// is where SART / Cimmino / gradient-descent steps fuse their vector work.
+//
+//  k_spmv_wave     rows from the compacted {row,start,end} list, SPMV_RPW rows per wavefront,
+//                  columns gathered from global memory (any CSR matrix).
+//  k_spmv_blocked  one workgroup per block of rows (blocked.hip): the block's column footprint is
+//                  first copied into LDS with coalesced segment loads, then entries read LDS
+//                  through a 16-bit local index -- no per-entry global gathers and 10 instead of
+//                  12 streamed bytes per FP64 entry.  Blocks whose footprint does not fit fall
+//                  back to global gathers inside the same launch.
+//
+// Column indices outside [0, ncols) contribute nothing (the reference can emit such indices,
+// see raytrace.hip).
 #pragma once
 
 #include "fair_internal.hpp"
@@ -28,11 +36,6 @@ struct EpiStore {
     __device__ __forceinline__ void operator()(int32_t r, V acc) const { y[r] = acc; }
 };
 
-template <class T>
-__device__ __forceinline__ T ldnt(const T *p) {
-    return __builtin_nontemporal_load(p);
-}
-
 template <class V>
 __device__ __forceinline__ V gather(const V *__restrict__ x, int32_t c, int32_t ncols) {
     return (uint32_t)c < (uint32_t)ncols ? x[c] : (V)0;
@@ -50,19 +53,80 @@ struct Vec2<float> {
 };
 typedef int32_t __attribute__((ext_vector_type(2))) int2v;
 
-// Rows per wavefront: the wave loads the descriptors of its rows with one load and walks them.
+// Rows per wavefront of the direct kernel: one descriptor load serves SPMV_RPW rows.
 constexpr int SPMV_RPW = 4;
-// Pair-chunks in flight per lane: 2 * 64 * SPMV_UNROLL entries of the row are requested before
-// the first gather (512 entries: a whole average fan-beam row at N = 512).
+// Pair-chunks in flight per lane (2 * 64 * SPMV_UNROLL entries of a row requested at once).
 constexpr int SPMV_UNROLL = 4;
 static_assert(CSR_PAD >= 2, "pair loads need a spare entry after the last row");
+
+// One row, columns gathered from global memory.  Pairs start at even entry indices, so FP64
+// pair loads are 16-byte and index pair loads 8-byte aligned; the pair straddling a row end
+// reads one entry of the next row (or of the padding) and masks it.
+template <class V>
+__device__ __forceinline__ V row_direct(int s, int e, int lane, const int32_t *__restrict__ JA,
+                                        const V *__restrict__ vA, const V *__restrict__ x,
+                                        int32_t ncols) {
+    typedef typename Vec2<V>::type V2;
+    V acc0 = 0, acc1 = 0;
+    for (int base = (s & ~1) + 2 * lane; base < e; base += 2 * WAVE * SPMV_UNROLL) {
+        V2 v[SPMV_UNROLL];
+        int2v c[SPMV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SPMV_UNROLL; ++u) {
+            const int k = base + 2 * WAVE * u;
+            if (k < e) {
+                v[u] = __builtin_nontemporal_load((const V2 *)(vA + k));
+                c[u] = __builtin_nontemporal_load((const int2v *)(JA + k));
+            } else {
+                v[u] = V2{0, 0};
+                c[u] = int2v{-1, -1};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SPMV_UNROLL; ++u) {
+            const int k = base + 2 * WAVE * u;
+            if (k >= s && k < e) acc0 += v[u].x * gather(x, c[u].x, ncols);
+            if (k + 1 < e) acc1 += v[u].y * gather(x, c[u].y, ncols);
+        }
+    }
+    return wave_sum(acc0 + acc1);
+}
+
+// One row, columns read from the LDS footprint through 16-bit local indices.
+template <class V>
+__device__ __forceinline__ V row_staged(int s, int e, int lane, const uint16_t *__restrict__ loc,
+                                        const V *__restrict__ vA, const V *lx) {
+    typedef typename Vec2<V>::type V2;
+    V acc0 = 0, acc1 = 0;
+    for (int base = (s & ~1) + 2 * lane; base < e; base += 2 * WAVE * SPMV_UNROLL) {
+        V2 v[SPMV_UNROLL];
+        uint32_t l[SPMV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SPMV_UNROLL; ++u) {
+            const int k = base + 2 * WAVE * u;
+            if (k < e) {
+                v[u] = __builtin_nontemporal_load((const V2 *)(vA + k));
+                l[u] = __builtin_nontemporal_load((const uint32_t *)(loc + k));
+            } else {
+                v[u] = V2{0, 0};
+                l[u] = 0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SPMV_UNROLL; ++u) {
+            const int k = base + 2 * WAVE * u;
+            if (k >= s && k < e) acc0 += v[u].x * lx[l[u] & 0xffffu];
+            if (k + 1 < e) acc1 += v[u].y * lx[l[u] >> 16];
+        }
+    }
+    return wave_sum(acc0 + acc1);
+}
 
 template <class V, class Src, class Epi>
 __global__ void __launch_bounds__(256) k_spmv_wave(const int4 *__restrict__ rows, int32_t nrows,
                                                    const int32_t *__restrict__ JA,
                                                    const V *__restrict__ vA, int32_t ncols,
                                                    Src src, Epi epi) {
-    typedef typename Vec2<V>::type V2;
     const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & (WAVE - 1);
     const int r0 = wv * SPMV_RPW;
@@ -74,31 +138,43 @@ __global__ void __launch_bounds__(256) k_spmv_wave(const int4 *__restrict__ rows
     if (lane < nr) d = rows[r0 + lane];
     for (int q = 0; q < nr; ++q) {
         const int row = __shfl(d.x, q, WAVE), s = __shfl(d.y, q, WAVE), e = __shfl(d.z, q, WAVE);
-        V acc0 = 0, acc1 = 0;
-        // pairs start at even indices: 16-byte aligned FP64 loads, 8-byte aligned index loads
-        for (int base = (s & ~1) + 2 * lane; base < e; base += 2 * WAVE * SPMV_UNROLL) {
-            V2 v[SPMV_UNROLL];
-            int2v c[SPMV_UNROLL];
-#pragma unroll
-            for (int u = 0; u < SPMV_UNROLL; ++u) {
-                const int k = base + 2 * WAVE * u;
-                if (k < e) {
-                    v[u] = __builtin_nontemporal_load((const V2 *)(vA + k));
-                    c[u] = __builtin_nontemporal_load((const int2v *)(JA + k));
-                } else {
-                    v[u] = V2{0, 0};
-                    c[u] = int2v{-1, -1};
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < SPMV_UNROLL; ++u) {
-                const int k = base + 2 * WAVE * u;
-                if (k >= s && k < e) acc0 += v[u].x * gather(x, c[u].x, ncols);
-                if (k + 1 < e) acc1 += v[u].y * gather(x, c[u].y, ncols);
-            }
-        }
-        const V acc = wave_sum(acc0 + acc1);
+        const V acc = row_direct<V>(s, e, lane, JA, vA, x, ncols);
         if (lane == 0) epi(row, acc);
+    }
+}
+
+template <class V, class Src, class Epi>
+__global__ void __launch_bounds__(256) k_spmv_blocked(const int4 *__restrict__ binfo,
+                                                      const int4 *__restrict__ rows,
+                                                      const int4 *__restrict__ segs,
+                                                      const uint16_t *__restrict__ loc,
+                                                      const int32_t *__restrict__ JA,
+                                                      const V *__restrict__ vA, int32_t ncols,
+                                                      int32_t zero_slot, Src src, Epi epi) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V *lx = (V *)smem_raw;
+    const V *__restrict__ x = src();
+    if (!x) return;
+    const int4 bi = binfo[blockIdx.x];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & (WAVE - 1);
+    if (bi.z >= 0) {
+        for (int s = bi.z + wv; s < bi.w; s += 4) {
+            const int4 sg = segs[s];
+            for (int t = lane; t < sg.y; t += WAVE) lx[sg.z + t] = x[sg.x + t];
+        }
+        if (threadIdx.x == 0) lx[zero_slot] = (V)0;
+        __syncthreads();
+        for (int q = bi.x + wv; q < bi.y; q += 4) {
+            const int4 ri = rows[q];
+            const V acc = row_staged<V>(ri.y, ri.z, lane, loc, vA, lx);
+            if (lane == 0) epi(ri.x, acc);
+        }
+    } else {
+        for (int q = bi.x + wv; q < bi.y; q += 4) {
+            const int4 ri = rows[q];
+            const V acc = row_direct<V>(ri.y, ri.z, lane, JA, vA, x, ncols);
+            if (lane == 0) epi(ri.x, acc);
+        }
     }
 }
 
@@ -111,10 +187,23 @@ inline void spmv_launch_raw(const int4 *rows, int32_t nrows, const int32_t *JA, 
                        s, rows, nrows, JA, vA, ncols, src, epi);
 }
 
+template <class V, class Src, class Epi>
+inline void spmv_launch_blocked(const Blocked &B, const int32_t *JA, const V *vA, int32_t ncols,
+                                Src src, Epi epi, hipStream_t s) {
+    if (B.nblocks <= 0) return;
+    const size_t shm = (size_t)B.lds_elems * sizeof(V);
+    hipLaunchKernelGGL((k_spmv_blocked<V, Src, Epi>), dim3((unsigned)B.nblocks), dim3(256), shm, s,
+                       (const int4 *)B.binfo.p, (const int4 *)B.rows.p, (const int4 *)B.segs.p,
+                       B.loc.p, JA, vA, ncols, B.lds_elems - 1, src, epi);
+}
+
 template <class Src, class Epi>
 inline void spmv_launch_src(const Csr &A, Src src, Epi epi) {
-    spmv_launch_raw<double, Src, Epi>((const int4 *)A.rows_nz.p, A.n_rows_nz, A.JA.p, A.vA.p, A.n,
-                                      src, epi, stream());
+    if (A.blk.ok)
+        spmv_launch_blocked<double, Src, Epi>(A.blk, A.JA.p, A.vA.p, A.n, src, epi, stream());
+    else
+        spmv_launch_raw<double, Src, Epi>((const int4 *)A.rows_nz.p, A.n_rows_nz, A.JA.p, A.vA.p,
+                                          A.n, src, epi, stream());
     launch_check("spmv");
 }
 
