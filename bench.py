@@ -33,6 +33,20 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def measured_traffic() -> int | None:
+    """HBM bytes per SpMV launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (profiles/spmv_n512_fan_fp64_hbm_traffic.md, line "traffic_bytes_per_launch: <int>")."""
+    path = os.path.join(ROOT, "profiles", "spmv_n512_fan_fp64_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(iters: int) -> dict | None:
     try:
         cores = len(os.sched_getaffinity(0))
@@ -79,6 +93,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 1 GiB fill and a 1 GiB sum (reference kernels for HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -153,6 +169,15 @@ def main() -> int:
     gbs = bytes_spmv / t_spmv / 1e9
     log(f"SpMV: {t_spmv * 1e6:.1f} us/launch, {gbs:.1f} GB/s ({gbs / HBM_PEAK_GBS:.1%} of peak)")
 
+    if a.calibrate:
+        # reference kernels of known HBM traffic for the FETCH_SIZE / WRITE_SIZE counters:
+        # a 1 GiB fill (writes 1 GiB) and a 1 GiB sum (reads 1 GiB; larger than the MALL)
+        big = torch.full((128 << 20,), 1.0, dtype=torch.float64, device=dev)
+        for _ in range(2):
+            float(big.sum())
+        torch.cuda.synchronize()
+        del big
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a.cpu_iters)
@@ -167,9 +192,9 @@ def main() -> int:
                    "global_batch": 1, "seq_len": 0, "parallelism": "single GPU",
                    "sart_gbs_per_iteration": round(bytes_it / (t_sart / a.steps) / 1e9, 2),
                    "matrix_build_s": round(t_build, 4)},
-        "roofline": {"kernel": "k_spmv_wave<double, SrcPtr<double>, EpiStore<double>>",
+        "roofline": {"kernel": "k_spmv_blocked<double, SrcPtr<double>, EpiStore<double>>",
                      "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "avg_launch_us": round(t_spmv * 1e6, 3), "algorithmic_bytes": bytes_spmv},
         "cpu_baseline": cpu,
     }

@@ -39,12 +39,14 @@ __device__ int block_exclusive_scan(int v, int *tsum, int *total) {
 
 // FILL = false: per block the number of segments (or -1 when the footprint + zero slot does
 // not fit in fmax entries).  FILL = true: write the segments at binfo.z and the entry slots.
+// A bucket's range is cut into segments of at most SEG_CHUNK entries, so that one thread
+// stages one segment with a short run of contiguous loads.
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_footprint(const int4 *__restrict__ binfo,
                                                    const int4 *__restrict__ rows,
                                                    const int32_t *__restrict__ JA, int32_t W,
                                                    int32_t NB, int32_t ncols, int32_t fmax,
-                                                   int32_t *nseg, int4 *segs, uint16_t *loc) {
+                                                   int32_t *nseg, int2 *segs, uint16_t *loc) {
     extern __shared__ int32_t sm[];
     int32_t *bmin = sm, *bmax = sm + NB, *boff = sm + 2 * NB;
     __shared__ int tsum[256];
@@ -73,8 +75,9 @@ __global__ void __launch_bounds__(256) k_footprint(const int4 *__restrict__ binf
     int len = 0, cnt = 0;
     for (int q = q0; q < q1; ++q)
         if (bmax[q] >= 0) {
-            len += bmax[q] - bmin[q] + 1;
-            ++cnt;
+            const int l = bmax[q] - bmin[q] + 1;
+            len += l;
+            cnt += (l + SEG_CHUNK - 1) / SEG_CHUNK;
         }
     int F, S;
     int lenoff = block_exclusive_scan(len, tsum, &F);
@@ -87,9 +90,12 @@ __global__ void __launch_bounds__(256) k_footprint(const int4 *__restrict__ binf
         if (bmax[q] >= 0) {
             const int l = bmax[q] - bmin[q] + 1;
             boff[q] = lenoff;
-            segs[bi.z + segoff] = make_int4(q * W + bmin[q], l, lenoff, 0);
+            for (int c0 = 0; c0 < l; c0 += SEG_CHUNK) {
+                const int cl = min(SEG_CHUNK, l - c0);
+                segs[bi.z + segoff] = make_int2(q * W + bmin[q] + c0, (cl << 16) | (lenoff + c0));
+                ++segoff;
+            }
             lenoff += l;
-            ++segoff;
         }
     }
     __syncthreads();
@@ -153,7 +159,7 @@ void build_blocked(Csr &A, const std::vector<int64_t> &key_of_row, int32_t rb, i
     DBuf<int32_t> nseg(nb);
     const size_t shm = sizeof(int32_t) * (3 * (size_t)NB + 1);
     hipLaunchKernelGGL(k_footprint<false>, dim3(nb), dim3(256), shm, stream(), (const int4 *)B.binfo.p,
-                       (const int4 *)B.rows.p, A.JA.p, W, NB, A.n, lds_elems, nseg.p, (int4 *)nullptr,
+                       (const int4 *)B.rows.p, A.JA.p, W, NB, A.n, lds_elems, nseg.p, (int2 *)nullptr,
                        (uint16_t *)nullptr);
     launch_check("footprint count");
     std::vector<int32_t> hn(nb);
@@ -174,12 +180,12 @@ void build_blocked(Csr &A, const std::vector<int64_t> &key_of_row, int32_t rb, i
     }
     FAIR_CHECK(total < INT32_MAX, FAIR_ERR_OVERFLOW, "too many footprint segments");
     FAIR_HIP(hipMemcpyAsync(B.binfo.p, binfo.data(), sizeof(int4) * nb, hipMemcpyHostToDevice, stream()));
-    B.segs.alloc(4 * (size_t)(total > 0 ? total : 1));
+    B.segs.alloc(2 * (size_t)(total > 0 ? total : 1));
     B.loc.alloc((size_t)A.nnz + CSR_PAD);
     FAIR_HIP(hipMemsetAsync(B.loc.p, 0, sizeof(uint16_t) * ((size_t)A.nnz + CSR_PAD), stream()));
     hipLaunchKernelGGL(k_footprint<true>, dim3(nb), dim3(256), shm, stream(), (const int4 *)B.binfo.p,
                        (const int4 *)B.rows.p, A.JA.p, W, NB, A.n, lds_elems, (int32_t *)nullptr,
-                       (int4 *)B.segs.p, B.loc.p);
+                       (int2 *)B.segs.p, B.loc.p);
     launch_check("footprint fill");
     FAIR_HIP(hipStreamSynchronize(stream()));
     B.nblocks = nb;
@@ -190,10 +196,17 @@ void build_blocked(Csr &A, const std::vector<int64_t> &key_of_row, int32_t rb, i
 
 }  // namespace
 
-void build_blocked_rays(Csr &A, int32_t img_n, int32_t p, int32_t RB, int32_t lds_elems) {
+// Rays are grouped by sinogram tile: DA consecutive angles x DJ consecutive detector
+// positions (rays of neighbouring angles through the same detector position nearly coincide,
+// so their pixel footprints overlap).
+void build_blocked_rays(Csr &A, int32_t img_n, int32_t p, int32_t DA, int32_t DJ, int32_t lds_elems) {
     std::vector<int64_t> key(A.m);
-    for (int32_t r = 0; r < A.m; ++r) key[r] = r / p;
-    build_blocked(A, key, RB, img_n, img_n, lds_elems);
+    const int64_t jblocks = (p + DJ - 1) / DJ;
+    for (int32_t r = 0; r < A.m; ++r) {
+        const int32_t th = r / p, j = r - th * p;
+        key[r] = (int64_t)(th / DA) * jblocks + j / DJ;
+    }
+    build_blocked(A, key, DA * DJ, img_n, img_n, lds_elems);
 }
 
 void build_blocked_tiles(Csr &T, int32_t img_n, int32_t p, int32_t nth, int32_t TX, int32_t TY,
@@ -212,13 +225,18 @@ void build_blocked_tiles(Csr &T, int32_t img_n, int32_t p, int32_t nth, int32_t 
 // block shape.
 void ensure_blocked(fair_csr *h) {
     if (h->g_n <= 0 || h->transposed || !env_int("FAIR_SPMV_STAGED", 1)) return;
-    const int32_t lds = env_int("FAIR_LDS", 6144);
-    if (!h->A.blk.ok) build_blocked_rays(h->A, h->g_n, h->g_p, env_int("FAIR_RB", 4), lds);
+    const bool built = !h->A.blk.ok || !h->have_T || !h->T.blk.ok;
+    if (!h->A.blk.ok)
+        build_blocked_rays(h->A, h->g_n, h->g_p, env_int("FAIR_DA", 4), env_int("FAIR_DJ", 4),
+                           env_int("FAIR_LDS_A", 6144));
     ensure_T(h);
     if (!h->T.blk.ok) {
-        const int tile = env_int("FAIR_TILE", 4);
-        build_blocked_tiles(h->T, h->g_n, h->g_p, h->g_nth, tile, tile, lds);
+        const int tx = env_int("FAIR_TX", 8), ty = env_int("FAIR_TY", 8);
+        build_blocked_tiles(h->T, h->g_n, h->g_p, h->g_nth, tx, ty, env_int("FAIR_LDS_T", 6144));
     }
+    if (env_int("FAIR_DEBUG", 0) && built)
+        fprintf(stderr, "[fair] staged SpMV: A %d blocks (%d direct), A^T %d blocks (%d direct)\n",
+                h->A.blk.nblocks, h->A.blk.n_direct, h->T.blk.nblocks, h->T.blk.n_direct);
 }
 
 }  // namespace fair

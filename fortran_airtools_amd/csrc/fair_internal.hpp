@@ -92,6 +92,8 @@ struct DBuf {
 // the host boundary is one-based), values vA (double).
 // Spare entries after JA / vA: the SpMV's aligned pair loads may touch one entry past a row.
 constexpr int CSR_PAD = 512;
+// Longest contiguous column run one thread stages into LDS (blocked SpMV).
+constexpr int SEG_CHUNK = 16;
 
 // Layout of the LDS-staged SpMV (blocked.hip): non-empty rows grouped into blocks whose
 // column footprint (per bucket of W consecutive columns, the [min,max] range touched) is
@@ -104,7 +106,7 @@ struct Blocked {
     int32_t n_direct = 0;       // blocks whose footprint does not fit (global gathers)
     DBuf<int32_t> binfo;        // int4 per block {row_begin, row_end, seg_begin, seg_end}
     DBuf<int32_t> rows;         // int4 row descriptors {row, start, end, 0}, grouped by block
-    DBuf<int32_t> segs;         // int4 {global column start, length, LDS offset, 0}
+    DBuf<int32_t> segs;         // int2 {global column start, length << 16 | LDS offset}
     DBuf<uint16_t> loc;         // per entry: LDS slot of its column (parallel to JA)
 };
 
@@ -150,9 +152,9 @@ void build_row_list(Csr &A);
 void alloc_entries(Csr &A, int64_t nnz);          // JA / vA with CSR_PAD zeroed spare entries
 void build_transpose(const Csr &A, Csr &T);      // T = CSR(A^T), rows sorted by column
 void sort_rows(Csr &A);                           // sort each row's entries by column
-// Blocked layouts: rays grouped RB at a time within one angle (columns bucketed by image
-// column), or pixels grouped into TX x TY tiles (columns bucketed by angle).
-void build_blocked_rays(Csr &A, int32_t img_n, int32_t p, int32_t RB, int32_t lds_elems);
+// Blocked layouts: rays grouped by DA x DJ sinogram tiles (columns bucketed by image column),
+// or pixels grouped into TX x TY image tiles (columns bucketed by angle).
+void build_blocked_rays(Csr &A, int32_t img_n, int32_t p, int32_t DA, int32_t DJ, int32_t lds_elems);
 void build_blocked_tiles(Csr &T, int32_t img_n, int32_t p, int32_t nth, int32_t TX, int32_t TY,
                          int32_t lds_elems);
 void ensure_blocked(fair_csr *h);

@@ -92,15 +92,16 @@ __device__ __forceinline__ V row_direct(int s, int e, int lane, const int32_t *_
     return wave_sum(acc0 + acc1);
 }
 
-// One row, columns read from the LDS footprint through 16-bit local indices.
+// One chunk (2 * 64 * SPMV_UNROLL entries) of a row held in registers: values and 16-bit
+// local indices of the LDS footprint.
 template <class V>
-__device__ __forceinline__ V row_staged(int s, int e, int lane, const uint16_t *__restrict__ loc,
-                                        const V *__restrict__ vA, const V *lx) {
-    typedef typename Vec2<V>::type V2;
-    V acc0 = 0, acc1 = 0;
-    for (int base = (s & ~1) + 2 * lane; base < e; base += 2 * WAVE * SPMV_UNROLL) {
-        V2 v[SPMV_UNROLL];
-        uint32_t l[SPMV_UNROLL];
+struct StagedChunk {
+    typename Vec2<V>::type v[SPMV_UNROLL];
+    uint32_t l[SPMV_UNROLL];
+
+    __device__ __forceinline__ void load(int base, int e, const uint16_t *__restrict__ loc,
+                                         const V *__restrict__ vA) {
+        typedef typename Vec2<V>::type V2;
 #pragma unroll
         for (int u = 0; u < SPMV_UNROLL; ++u) {
             const int k = base + 2 * WAVE * u;
@@ -112,6 +113,9 @@ __device__ __forceinline__ V row_staged(int s, int e, int lane, const uint16_t *
                 l[u] = 0;
             }
         }
+    }
+
+    __device__ __forceinline__ void use(int base, int s, int e, const V *lx, V &acc0, V &acc1) const {
 #pragma unroll
         for (int u = 0; u < SPMV_UNROLL; ++u) {
             const int k = base + 2 * WAVE * u;
@@ -119,8 +123,7 @@ __device__ __forceinline__ V row_staged(int s, int e, int lane, const uint16_t *
             if (k + 1 < e) acc1 += v[u].y * lx[l[u] >> 16];
         }
     }
-    return wave_sum(acc0 + acc1);
-}
+};
 
 template <class V, class Src, class Epi>
 __global__ void __launch_bounds__(256) k_spmv_wave(const int4 *__restrict__ rows, int32_t nrows,
@@ -143,14 +146,18 @@ __global__ void __launch_bounds__(256) k_spmv_wave(const int4 *__restrict__ rows
     }
 }
 
+// Workgroup of SPMV_BLOCK_THREADS: each wavefront first requests the first chunk of its first
+// row (independent of LDS), then all threads stage the footprint (one thread per segment of at
+// most SEG_CHUNK contiguous columns), and after the barrier the prefetched chunk is consumed,
+// so the matrix stream and the footprint loads are in flight together.
+constexpr int SPMV_BLOCK_THREADS = 512;
+
 template <class V, class Src, class Epi>
-__global__ void __launch_bounds__(256) k_spmv_blocked(const int4 *__restrict__ binfo,
-                                                      const int4 *__restrict__ rows,
-                                                      const int4 *__restrict__ segs,
-                                                      const uint16_t *__restrict__ loc,
-                                                      const int32_t *__restrict__ JA,
-                                                      const V *__restrict__ vA, int32_t ncols,
-                                                      int32_t zero_slot, Src src, Epi epi) {
+__global__ void __launch_bounds__(SPMV_BLOCK_THREADS) k_spmv_blocked(
+    const int4 *__restrict__ binfo, const int4 *__restrict__ rows, const int2 *__restrict__ segs,
+    const uint16_t *__restrict__ loc, const int32_t *__restrict__ JA, const V *__restrict__ vA,
+    int32_t ncols, int32_t zero_slot, Src src, Epi epi) {
+    constexpr int NW = SPMV_BLOCK_THREADS / WAVE;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V *lx = (V *)smem_raw;
     const V *__restrict__ x = src();
@@ -158,19 +165,42 @@ __global__ void __launch_bounds__(256) k_spmv_blocked(const int4 *__restrict__ b
     const int4 bi = binfo[blockIdx.x];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & (WAVE - 1);
     if (bi.z >= 0) {
-        for (int s = bi.z + wv; s < bi.w; s += 4) {
-            const int4 sg = segs[s];
-            for (int t = lane; t < sg.y; t += WAVE) lx[sg.z + t] = x[sg.x + t];
+        int q = bi.x + wv;
+        int4 ri = make_int4(0, 0, 0, 0);
+        StagedChunk<V> ch;
+        if (q < bi.y) {
+            ri = rows[q];
+            ch.load((ri.y & ~1) + 2 * lane, ri.z, loc, vA);
+        }
+        for (int s = bi.z + (int)threadIdx.x; s < bi.w; s += SPMV_BLOCK_THREADS) {
+            const int2 sg = segs[s];
+            const int len = sg.y >> 16;
+            const V *__restrict__ xs = x + sg.x;
+            V *d = lx + (sg.y & 0xffff);
+#pragma unroll 4
+            for (int t = 0; t < len; ++t) d[t] = xs[t];
         }
         if (threadIdx.x == 0) lx[zero_slot] = (V)0;
         __syncthreads();
-        for (int q = bi.x + wv; q < bi.y; q += 4) {
-            const int4 ri = rows[q];
-            const V acc = row_staged<V>(ri.y, ri.z, lane, loc, vA, lx);
+        bool first = true;
+        for (; q < bi.y; q += NW) {
+            if (!first) ri = rows[q];
+            V acc0 = 0, acc1 = 0;
+            int base = (ri.y & ~1) + 2 * lane;
+            if (first) {
+                ch.use(base, ri.y, ri.z, lx, acc0, acc1);
+                base += 2 * WAVE * SPMV_UNROLL;
+                first = false;
+            }
+            for (; base < ri.z; base += 2 * WAVE * SPMV_UNROLL) {
+                ch.load(base, ri.z, loc, vA);
+                ch.use(base, ri.y, ri.z, lx, acc0, acc1);
+            }
+            const V acc = wave_sum(acc0 + acc1);
             if (lane == 0) epi(ri.x, acc);
         }
     } else {
-        for (int q = bi.x + wv; q < bi.y; q += 4) {
+        for (int q = bi.x + wv; q < bi.y; q += NW) {
             const int4 ri = rows[q];
             const V acc = row_direct<V>(ri.y, ri.z, lane, JA, vA, x, ncols);
             if (lane == 0) epi(ri.x, acc);
@@ -192,8 +222,8 @@ inline void spmv_launch_blocked(const Blocked &B, const int32_t *JA, const V *vA
                                 Src src, Epi epi, hipStream_t s) {
     if (B.nblocks <= 0) return;
     const size_t shm = (size_t)B.lds_elems * sizeof(V);
-    hipLaunchKernelGGL((k_spmv_blocked<V, Src, Epi>), dim3((unsigned)B.nblocks), dim3(256), shm, s,
-                       (const int4 *)B.binfo.p, (const int4 *)B.rows.p, (const int4 *)B.segs.p,
+    hipLaunchKernelGGL((k_spmv_blocked<V, Src, Epi>), dim3((unsigned)B.nblocks), dim3(SPMV_BLOCK_THREADS), shm, s,
+                       (const int4 *)B.binfo.p, (const int4 *)B.rows.p, (const int2 *)B.segs.p,
                        B.loc.p, JA, vA, ncols, B.lds_elems - 1, src, epi);
 }
 
