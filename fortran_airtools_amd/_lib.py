@@ -66,6 +66,12 @@ SIGNATURES = {
     "fair_sirt": [_P, _I32, _P, _I32, _P, C.POINTER(Opts), _P],
     "fair_sirt_dev": [_P, _I32, _P, _I32, _P, C.POINTER(Opts)],
     "fair_sirt_prepare": [_P, _I32],
+    "fair_sirt_f32": [_P, _I32, _P, _I32, _P, C.POINTER(Opts), _P],
+    "fair_sirt_f32_dev": [_P, _I32, _P, _I32, _P, C.POINTER(Opts)],
+    "fair_sirt_batch_dev": [_P, _I32, _P, _I32, _I32, _P, C.POINTER(Opts), _I32],
+    "fair_batch_relerr_dev": [_P, _P, _I32, _I32, _P],
+    "fair_sirt_batch": [_P, _I32, _P, _I32, _I32, _P, C.POINTER(Opts), _I32, _P],
+    "fair_autotune_batch": [_P, _I32, _P],
     "fair_normest": [_P, _P, _D, _I32, _P],
     "fair_gradient_descent": [_P, _P, _I32, _P, _D, _D, _D, C.POINTER(Opts), _P, _P],
     "fair_art": [_P, _I32, _P, _I32, _P, _P, _P, _I32, C.POINTER(Opts), _P, _I64, _P],

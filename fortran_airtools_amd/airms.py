@@ -86,6 +86,45 @@ def sirt_dev(sirt_method: str, A: CSRMatrix, d_b: int, maxits: int, d_x: int,
                               C.c_void_p(d_x), C.byref(o)))
 
 
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None and a.shape[0] != n:
+        raise FairError(FAIR_ERR_ARG, "Vector input size incorrect.")
+    return a
+
+
+def sirt_f32(sirt_method: str, A: CSRMatrix, b, Kin, x0=None, options: Opts | None = None) -> np.ndarray:
+    """Precision variant of sirt: FP32 matrix values, iterate and accumulation, FP64 weights."""
+    if sirt_method not in SIRT_METHODS:
+        raise FairError(FAIR_ERR_METHOD, "Not a recognised SIRT method.")
+    b = _f32(b, A.m)
+    x0a = None if x0 is None else _f32(x0, A.n)
+    out = np.empty(A.n, np.float32)
+    o = _opts(options)
+    check(lib().fair_sirt_f32(A.handle, SIRT_METHODS[sirt_method], _ptr(b), _maxval(Kin), _ptr(x0a),
+                              C.byref(o), _ptr(out)))
+    return out
+
+
+def sirt_batch(sirt_method: str, A: CSRMatrix, B, Kin, X0=None, options: Opts | None = None,
+               use_graph: bool = False) -> np.ndarray:
+    """Batched FP32 sirt over the columns of B (shape (m, k)); returns X of shape (n, k)."""
+    if sirt_method not in SIRT_METHODS:
+        raise FairError(FAIR_ERR_METHOD, "Not a recognised SIRT method.")
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    if B.ndim != 2 or B.shape[0] != A.m:
+        raise FairError(FAIR_ERR_ARG, "B must have shape (m, k)")
+    k = B.shape[1]
+    X0a = None if X0 is None else np.ascontiguousarray(X0, dtype=np.float32)
+    if X0a is not None and X0a.shape != (A.n, k):
+        raise FairError(FAIR_ERR_ARG, "X0 must have shape (n, k)")
+    out = np.empty((A.n, k), np.float32)
+    o = _opts(options)
+    check(lib().fair_sirt_batch(A.handle, SIRT_METHODS[sirt_method], _ptr(B), k, _maxval(Kin), _ptr(X0a),
+                                C.byref(o), int(bool(use_graph)), _ptr(out)))
+    return out
+
+
 def normest(A: CSRMatrix, tolerance: float = 1e-5, max_its: int = 100, v0=None, seed: int = 0) -> float:
     """normest (AIRMs.f90:1129-1173): power iteration on A^T A from v0 (uniform [0,1) draws,
     seeded here when not supplied)."""

@@ -1,0 +1,471 @@
+// batch.hip -- precision variants and batched SIRT.
+//
+// FP32 single right-hand side: the same blocked SpMV kernels instantiated for float (FP32 matrix
+// values, FP32 iterate, FP32 accumulation); the SIRT weights M, D stay FP64 and the per-row /
+// per-column scaling is applied in FP64 before rounding the result to FP32.
+//
+// Batched SIRT (k right-hand sides, e.g. k slices sharing one geometry): X is n x k and B, R are
+// m x k, row-major (the k values of one pixel / ray are contiguous).  k_spmm handles one matrix
+// row for SPMM_KC right-hand sides per wavefront: the 64 lanes are 64/SPMM_KC entry slots x
+// SPMM_KC right-hand sides, so every gathered matrix entry pulls SPMM_KC contiguous floats of X.
+// The chunk of right-hand sides a block works on is chosen from the block index so that the
+// blocks resident on one XCD (blocks are dealt round-robin to the 8 XCDs) share one chunk and
+// its slice of X stays in that XCD's L2.  The iteration loop can be captured once into a HIP
+// graph and replayed.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <tuple>
+
+#include "fair_internal.hpp"
+#include "spmv.hpp"
+
+namespace fair {
+namespace {
+
+__global__ void k_to_f32(const double *a, float *b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+void make_f32(Csr &A) {
+    if (A.vA32.p) return;
+    A.vA32.alloc((size_t)A.nnz + CSR_PAD);
+    FAIR_HIP(hipMemsetAsync(A.vA32.p + A.nnz, 0, sizeof(float) * CSR_PAD, stream()));
+    if (A.nnz > 0)
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A.nnz, 256)), dim3(256), 0, stream(), A.vA.p, A.vA32.p, A.nnz);
+    launch_check("fp32 values");
+}
+
+struct BoxF {
+    int upb, lwrb;
+    float ub, lb;
+    __device__ __forceinline__ float apply(float v) const {
+        if (upb) v = fmin_ref(v, ub);
+        if (lwrb) v = fmax_ref(v, lb);
+        return v;
+    }
+};
+
+BoxF make_boxf(const fair_opts *o) {
+    BoxF b{0, 0, 0.f, 1.f};
+    if (o) {
+        b.upb = o->upb != 0;
+        b.lwrb = o->lwrb != 0;
+        b.ub = (float)o->ubound;
+        b.lb = (float)o->lbound;
+    }
+    return b;
+}
+
+// r = (b - Ax) / M with M in FP64
+struct EpiResidualF {
+    const float *b;
+    const double *M;
+    float *r;
+    __device__ __forceinline__ void operator()(int32_t i, float acc) const {
+        r[i] = (float)(((double)b[i] - (double)acc) / M[i]);
+    }
+};
+
+// x_new = box(x + relax * (A^T r) / D), scaling in FP64
+struct EpiSirtUpdateF {
+    const float *x;
+    const double *D;
+    float *xn;
+    double relax;
+    BoxF box;
+    __device__ __forceinline__ void operator()(int32_t j, float acc) const {
+        double u = relax * (double)acc;
+        if (D) u = u / D[j];
+        xn[j] = box.apply((float)((double)x[j] + u));
+    }
+};
+
+template <class Src, class Epi>
+void spmv_f32(const Csr &A, Src src, Epi epi) {
+    if (A.blk.ok)
+        spmv_launch_blocked<float, Src, Epi>(A.blk, A.JA.p, A.vA32.p, A.n, src, epi, stream());
+    else
+        spmv_launch_raw<float, Src, Epi>((const int4 *)A.rows_nz.p, A.n_rows_nz, A.JA.p, A.vA32.p, A.n,
+                                         src, epi, stream());
+    launch_check("spmv fp32");
+}
+
+__global__ void k_copy_f32(const float *a, float *b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+void check_handle(const fair_csr *A) { FAIR_CHECK(A != nullptr, FAIR_ERR_ARG, "null matrix handle"); }
+
+// ---- batched: SpMM ----------------------------------------------------------------------------
+// KC right-hand sides per wavefront; 64/KC matrix entries side by side.  Blocks are numbered
+// phase-major: consecutive blocks b, b+1, ... go to XCDs b % 8, and phase ph covers chunks
+// 8*ph .. 8*ph+7 with chunk 8*ph + (b % 8) on XCD b % 8, so while a phase runs every XCD
+// streams the matrix against one chunk of X whose slice (n * KC floats) stays in its L2.
+constexpr int SPMM_UNROLL = 4;
+constexpr int NXCD = 8;
+
+template <int KC, class Epi>
+__global__ void __launch_bounds__(256) k_spmm(const int4 *__restrict__ rows, int32_t nrows,
+                                              const int32_t *__restrict__ JA,
+                                              const float *__restrict__ vA, int32_t ncols,
+                                              const float *__restrict__ X, int32_t k, int32_t nchunks,
+                                              int32_t nrb, Epi epi) {
+    constexpr int SLOTS = WAVE / KC;
+    const int b = blockIdx.x;
+    const int phase = b / (NXCD * nrb);
+    const int chunk = phase * NXCD + (b % NXCD);
+    if (chunk >= nchunks) return;
+    const int rb = (b / NXCD) % nrb;
+    const int wv = rb * 4 + (threadIdx.x >> 6);
+    if (wv >= nrows) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int slot = lane / KC, j = lane % KC;
+    const int col0 = chunk * KC + j;
+    const bool active = col0 < k;
+    const int4 ri = rows[wv];
+    float acc = 0.f;
+    for (int base = ri.y + slot; base < ri.z; base += SLOTS * SPMM_UNROLL) {
+        int32_t c[SPMM_UNROLL];
+        float v[SPMM_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SPMM_UNROLL; ++u) {
+            const int q = base + SLOTS * u;
+            c[u] = q < ri.z ? JA[q] : -1;
+            v[u] = q < ri.z ? vA[q] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < SPMM_UNROLL; ++u)
+            if ((uint32_t)c[u] < (uint32_t)ncols && active) acc += v[u] * X[(int64_t)c[u] * k + col0];
+    }
+#pragma unroll
+    for (int off = KC; off < WAVE; off <<= 1) acc += __shfl_xor(acc, off, WAVE);
+    if (slot == 0 && active) epi(ri.x, col0, acc);
+}
+
+template <class Epi>
+__global__ void k_spmm_empty(const int32_t *__restrict__ rows, int32_t nrows, int32_t k, Epi epi) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nrows * k) return;
+    epi(rows[t / k], (int)(t % k), 0.f);
+}
+
+struct EpiStoreB {
+    float *Y;
+    int32_t k;
+    __device__ __forceinline__ void operator()(int32_t i, int j, float acc) const { Y[(int64_t)i * k + j] = acc; }
+};
+
+struct EpiResidualB {
+    const float *B;
+    const double *M;
+    float *R;
+    int32_t k;
+    __device__ __forceinline__ void operator()(int32_t i, int j, float acc) const {
+        const int64_t q = (int64_t)i * k + j;
+        R[q] = (float)(((double)B[q] - (double)acc) / M[i]);
+    }
+};
+
+struct EpiUpdateB {
+    const float *X;
+    const double *D;
+    float *Xn;
+    int32_t k;
+    double relax;
+    BoxF box;
+    __device__ __forceinline__ void operator()(int32_t p, int j, float acc) const {
+        const int64_t q = (int64_t)p * k + j;
+        double u = relax * (double)acc;
+        if (D) u = u / D[p];
+        Xn[q] = box.apply((float)((double)X[q] + u));
+    }
+};
+
+template <int KC, class Epi>
+void spmm_launch_kc(const Csr &A, const float *X, int32_t k, Epi epi, hipStream_t s) {
+    const int nchunks = (k + KC - 1) / KC;
+    const int nrb = (A.n_rows_nz + 3) / 4;
+    const int phases = (nchunks + NXCD - 1) / NXCD;
+    const int64_t blocks = (int64_t)phases * NXCD * nrb;
+    hipLaunchKernelGGL((k_spmm<KC, Epi>), dim3((unsigned)blocks), dim3(256), 0, s, (const int4 *)A.rows_nz.p,
+                       A.n_rows_nz, A.JA.p, A.vA32.p, A.n, X, k, nchunks, nrb, epi);
+}
+
+// Right-hand sides per wavefront: 8, 16 or 32 (chosen by fair_autotune_batch, default 16).
+template <class Epi>
+void spmm_launch(const Csr &A, const float *X, int32_t k, int kc, Epi epi, hipStream_t s) {
+    if (A.n_rows_nz <= 0 || k <= 0) return;
+    if (kc == 8) spmm_launch_kc<8>(A, X, k, epi, s);
+    else if (kc == 32) spmm_launch_kc<32>(A, X, k, epi, s);
+    else spmm_launch_kc<16>(A, X, k, epi, s);
+}
+
+// Cached HIP graphs of whole batched runs, keyed by the run's arguments.
+struct GraphKey {
+    const void *h, *B, *X;
+    int32_t method, k, maxits, upb, lwrb;
+    double relax, ub, lb;
+    bool operator<(const GraphKey &o) const {
+        return std::tie(h, B, X, method, k, maxits, upb, lwrb, relax, ub, lb) <
+               std::tie(o.h, o.B, o.X, o.method, o.k, o.maxits, o.upb, o.lwrb, o.relax, o.ub, o.lb);
+    }
+};
+std::map<GraphKey, hipGraphExec_t> g_graphs;
+hipStream_t g_capture = nullptr;
+
+}  // namespace
+
+// Cached SIRT weights and FP32 values (shared with methods.hip's FP64 set-up).
+void sirt_setup_public(fair_csr *h, int method);
+
+namespace {
+
+void batch_iterations(fair_csr *h, int method, const float *dB, int32_t k, int32_t maxits, float *dX,
+                      float *dR, float *dXn, const fair_opts *o, hipStream_t s) {
+    const Csr &A = h->A, &T = h->T;
+    const double *M = h->M[method].p;
+    const double *D = method == FAIR_SART ? h->D.p : nullptr;
+    const double relax = o ? o->relaxpar : 1.0;
+    const BoxF box = make_boxf(o);
+    float *cur = dX, *nxt = dXn;
+    const int kca = h->spmm_kc[0], kct = h->spmm_kc[1];
+    for (int it = 0; it < maxits; ++it) {
+        spmm_launch(A, cur, k, kca, EpiResidualB{dB, M, dR, k}, s);
+        EpiUpdateB up{cur, D, nxt, k, relax, box};
+        spmm_launch(T, dR, k, kct, up, s);
+        if (T.n_rows_empty > 0) {
+            const int64_t t = (int64_t)T.n_rows_empty * k;
+            hipLaunchKernelGGL((k_spmm_empty<EpiUpdateB>), dim3(grid_for(t, 256)), dim3(256), 0, s,
+                               T.rows_empty.p, T.n_rows_empty, k, up);
+        }
+        std::swap(cur, nxt);
+    }
+    if (cur != dX) {
+        const int64_t t = (int64_t)A.n * k;
+        hipLaunchKernelGGL(k_copy_f32, dim3(grid_for(t, 256)), dim3(256), 0, s, cur, dX, t);
+    }
+}
+
+}  // namespace
+}  // namespace fair
+
+using namespace fair;
+
+extern "C" int fair_sirt_f32_dev(fair_csr *h, int32_t method, const float *d_b, int32_t maxits, float *d_x,
+                                 const fair_opts *o) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(maxits >= 0, FAIR_ERR_ARG, "negative iteration count");
+        sirt_setup_public(h, method);
+        make_f32(h->A);
+        make_f32(h->T);
+        const Csr &A = h->A, &T = h->T;
+        const int32_t m = A.m, n = A.n;
+        if (h->ws_r32.n < (size_t)(m > 0 ? m : 1)) h->ws_r32.alloc(m > 0 ? m : 1);
+        if (h->ws_x32.n < (size_t)(n > 0 ? n : 1)) h->ws_x32.alloc(n > 0 ? n : 1);
+        const double *M = h->M[method].p;
+        const double *D = method == FAIR_SART ? h->D.p : nullptr;
+        const double relax = o ? o->relaxpar : 1.0;
+        const BoxF box = make_boxf(o);
+        float *cur = d_x, *nxt = h->ws_x32.p, *r = h->ws_r32.p;
+        for (int it = 0; it < maxits; ++it) {
+            spmv_f32(A, SrcPtr<float>{cur}, EpiResidualF{d_b, M, r});
+            EpiSirtUpdateF up{cur, D, nxt, relax, box};
+            spmv_f32(T, SrcPtr<float>{r}, up);
+            if (T.n_rows_empty > 0)
+                hipLaunchKernelGGL((k_empty_rows<float, SrcPtr<float>, EpiSirtUpdateF>),
+                                   dim3(grid_for(T.n_rows_empty, 256)), dim3(256), 0, stream(),
+                                   T.rows_empty.p, T.n_rows_empty, SrcPtr<float>{r}, up);
+            std::swap(cur, nxt);
+        }
+        if (cur != d_x)
+            hipLaunchKernelGGL(k_copy_f32, dim3(grid_for(n, 256)), dim3(256), 0, stream(), cur, d_x, (int64_t)n);
+        launch_check("sirt fp32");
+    });
+}
+
+extern "C" int fair_sirt_f32(fair_csr *h, int32_t method, const float *b, int32_t maxits, const float *x0,
+                             const fair_opts *o, float *xout) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(b && xout, FAIR_ERR_ARG, "null argument");
+        const int32_t m = h->A.m, n = h->A.n;
+        DBuf<float> db(m > 0 ? m : 1), dx(n > 0 ? n : 1);
+        db.upload(b, m);
+        if (x0) dx.upload(x0, n);
+        else FAIR_HIP(hipMemsetAsync(dx.p, 0, sizeof(float) * (n > 0 ? n : 1), stream()));
+        int st = fair_sirt_f32_dev(h, method, db.p, maxits, dx.p, o);
+        if (st != FAIR_OK) throw Error{st, fair_last_error()};
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        dx.download(xout, n);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+    });
+}
+
+extern "C" int fair_sirt_batch_dev(fair_csr *h, int32_t method, const float *d_B, int32_t k, int32_t maxits,
+                                   float *d_X, const fair_opts *o, int32_t use_graph) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(k >= 1 && maxits >= 0, FAIR_ERR_ARG, "k must be >= 1 and maxits >= 0");
+        sirt_setup_public(h, method);
+        make_f32(h->A);
+        make_f32(h->T);
+        const int32_t m = h->A.m, n = h->A.n;
+        const size_t need_r = (size_t)m * k, need_x = (size_t)n * k;
+        if (h->ws_rb.n < need_r) h->ws_rb.alloc(need_r);
+        if (h->ws_xb.n < need_x) h->ws_xb.alloc(need_x);
+        if (!use_graph) {
+            batch_iterations(h, method, d_B, k, maxits, d_X, h->ws_rb.p, h->ws_xb.p, o, stream());
+            launch_check("sirt batch");
+            return;
+        }
+        // graph path: capture the whole run once on a private stream, replay afterwards
+        if (!g_capture) FAIR_HIP(hipStreamCreateWithFlags(&g_capture, hipStreamNonBlocking));
+        GraphKey key{h, d_B, d_X, method, k, maxits, o ? o->upb : 0, o ? o->lwrb : 0,
+                     o ? o->relaxpar : 1.0, o ? o->ubound : 0.0, o ? o->lbound : 1.0};
+        auto it = g_graphs.find(key);
+        if (it == g_graphs.end()) {
+            FAIR_HIP(hipStreamSynchronize(stream()));
+            hipGraph_t graph;
+            FAIR_HIP(hipStreamBeginCapture(g_capture, hipStreamCaptureModeThreadLocal));
+            batch_iterations(h, method, d_B, k, maxits, d_X, h->ws_rb.p, h->ws_xb.p, o, g_capture);
+            hipError_t le = hipGetLastError();
+            FAIR_HIP(hipStreamEndCapture(g_capture, &graph));
+            FAIR_HIP(le);
+            hipGraphExec_t exec;
+            FAIR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            FAIR_HIP(hipGraphDestroy(graph));
+            it = g_graphs.emplace(key, exec).first;
+            h->graphs_used = true;
+        }
+        // order the replay after the caller's work on the library stream
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        FAIR_HIP(hipGraphLaunch(it->second, g_capture));
+        FAIR_HIP(hipStreamSynchronize(g_capture));
+    });
+}
+
+// Release the graphs that captured this handle (called by fair_csr_free).
+namespace fair {
+void drop_graphs(const fair_csr *h) {
+    for (auto it = g_graphs.begin(); it != g_graphs.end();) {
+        if (it->first.h == h) {
+            (void)hipGraphExecDestroy(it->second);
+            it = g_graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+}  // namespace fair
+
+// Per right-hand side relative error ||X_j - Xref_j|| / ||Xref_j|| with FP64 sums (X: n x k FP32,
+// Xref: n x k FP64, both row-major).
+namespace {
+__global__ void __launch_bounds__(256) k_batch_err(const float *X, const double *Xr, int32_t n, int32_t k,
+                                                  double *num, double *den) {
+    const int j = blockIdx.y;
+    double a = 0.0, b = 0.0;
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
+        const double r = Xr[p * k + j], d = (double)X[p * k + j] - r;
+        a += d * d;
+        b += r * r;
+    }
+    a = fair::wave_sum(a);
+    b = fair::wave_sum(b);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&num[j], a);
+        atomicAdd(&den[j], b);
+    }
+}
+}  // namespace
+
+extern "C" int fair_batch_relerr_dev(const float *d_X, const double *d_Xref, int32_t n, int32_t k, double *err) {
+    return guard([&] {
+        FAIR_CHECK(d_X && d_Xref && err && n >= 0 && k >= 1, FAIR_ERR_ARG, "bad arguments");
+        DBuf<double> num(k), den(k);
+        FAIR_HIP(hipMemsetAsync(num.p, 0, sizeof(double) * k, stream()));
+        FAIR_HIP(hipMemsetAsync(den.p, 0, sizeof(double) * k, stream()));
+        hipLaunchKernelGGL(k_batch_err, dim3(64, k), dim3(256), 0, stream(), d_X, d_Xref, n, k, num.p, den.p);
+        launch_check("batch error");
+        std::vector<double> a(k), b(k);
+        num.download(a.data(), k);
+        den.download(b.data(), k);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        for (int j = 0; j < k; ++j) err[j] = std::sqrt(a[j]) / (b[j] > 0 ? std::sqrt(b[j]) : 1.0);
+    });
+}
+
+// Autotuner of the batched SpMM: times each candidate width on this matrix and k with HIP
+// events (one A*X and one A^T*R launch each, after a warm-up) and keeps the fastest per
+// direction in the handle.
+extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *kc_out) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(k >= 1, FAIR_ERR_ARG, "k must be >= 1");
+        sirt_setup_public(h, FAIR_SART);
+        make_f32(h->A);
+        make_f32(h->T);
+        const Csr &A = h->A, &T = h->T;
+        DBuf<float> X((size_t)A.n * k), R((size_t)A.m * k), Y((size_t)std::max(A.m, A.n) * k);
+        FAIR_HIP(hipMemsetAsync(X.p, 0, X.bytes(), stream()));
+        FAIR_HIP(hipMemsetAsync(R.p, 0, R.bytes(), stream()));
+        hipEvent_t e0, e1;
+        FAIR_HIP(hipEventCreate(&e0));
+        FAIR_HIP(hipEventCreate(&e1));
+        const int cands[3] = {8, 16, 32};
+        for (int dir = 0; dir < 2; ++dir) {
+            const Csr &M = dir == 0 ? A : T;
+            const float *src = dir == 0 ? X.p : R.p;
+            float best = 1e30f;
+            int best_kc = 16;
+            for (int kc : cands) {
+                auto run = [&] { spmm_launch(M, src, k, kc, EpiStoreB{Y.p, k}, stream()); };
+                run();
+                FAIR_HIP(hipEventRecord(e0, stream()));
+                run();
+                run();
+                FAIR_HIP(hipEventRecord(e1, stream()));
+                FAIR_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    best_kc = kc;
+                }
+            }
+            h->spmm_kc[dir] = best_kc;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        launch_check("autotune batch");
+        if (kc_out) {
+            kc_out[0] = h->spmm_kc[0];
+            kc_out[1] = h->spmm_kc[1];
+        }
+    });
+}
+
+extern "C" int fair_sirt_batch(fair_csr *h, int32_t method, const float *B, int32_t k, int32_t maxits,
+                               const float *X0, const fair_opts *o, int32_t use_graph, float *Xout) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(B && Xout && k >= 1, FAIR_ERR_ARG, "bad arguments");
+        const size_t mk = (size_t)h->A.m * k, nk = (size_t)h->A.n * k;
+        DBuf<float> dB(mk > 0 ? mk : 1), dX(nk > 0 ? nk : 1);
+        dB.upload(B, mk);
+        if (X0) dX.upload(X0, nk);
+        else FAIR_HIP(hipMemsetAsync(dX.p, 0, sizeof(float) * (nk > 0 ? nk : 1), stream()));
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        int st = fair_sirt_batch_dev(h, method, dB.p, k, maxits, dX.p, o, use_graph);
+        if (st != FAIR_OK) throw Error{st, fair_last_error()};
+        // a graph keyed by these temporary buffers would dangle: release it
+        if (use_graph) drop_graphs(h);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        dX.download(Xout, nk);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+    });
+}

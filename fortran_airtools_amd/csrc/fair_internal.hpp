@@ -115,6 +115,7 @@ struct Csr {
     int64_t nnz = 0;
     DBuf<int32_t> IA, JA;
     DBuf<double> vA;
+    DBuf<float> vA32;        // FP32 copy of the values (precision variants), made on demand
     DBuf<int32_t> rows_nz;   // compacted list of non-empty rows, int4 {row, start, end, 0}
     int32_t n_rows_nz = -1;  // -1: list not built
     DBuf<int32_t> rows_empty;  // ids of the empty rows
@@ -140,8 +141,13 @@ struct fair_csr {
     fair::DBuf<double> M[2], D;
     bool w_ok[2] = {false, false};
     fair::DBuf<double> ws_r, ws_x;
+    fair::DBuf<float> ws_r32, ws_x32, ws_rb, ws_xb;   // FP32 / batched workspaces
+    bool graphs_used = false;
+    int spmm_kc[2] = {16, 16};   // batched SpMM right-hand sides per wavefront (A, A^T)
     void drop_cache() {
         w_ok[0] = w_ok[1] = false;
+        A.vA32.release();
+        T.vA32.release();
     }
 };
 
@@ -158,6 +164,7 @@ void build_blocked_rays(Csr &A, int32_t img_n, int32_t p, int32_t DA, int32_t DJ
 void build_blocked_tiles(Csr &T, int32_t img_n, int32_t p, int32_t nth, int32_t TX, int32_t TY,
                          int32_t lds_elems);
 void ensure_blocked(fair_csr *h);
+void drop_graphs(const fair_csr *h);
 void ensure_T(fair_csr *h);
 void invalidate_T(fair_csr *h);
 

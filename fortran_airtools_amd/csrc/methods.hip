@@ -322,6 +322,9 @@ void sirt_run(fair_csr *h, int method, const double *d_b, int maxits, double *d_
 }
 
 }  // namespace
+
+void sirt_setup_public(fair_csr *h, int method) { sirt_setup(h, method); }
+
 }  // namespace fair
 
 using namespace fair;

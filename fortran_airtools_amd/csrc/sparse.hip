@@ -501,6 +501,7 @@ extern "C" int fair_csr_free(fair_csr *h) {
     return guard([&] {
         if (h) {
             (void)hipStreamSynchronize(stream());
+            if (h->graphs_used) drop_graphs(h);
             delete h;
         }
     });

@@ -121,6 +121,28 @@ int fair_sirt_dev(fair_csr *A, int32_t method, const double *d_b, int32_t maxits
  * A's rows as the reference's double transpose does), so later fair_sirt_dev calls only
  * iterate. */
 int fair_sirt_prepare(fair_csr *A, int32_t method);
+/* Precision variant of sirt (AIRMs.f90:997-1127): FP32 matrix values, iterate and accumulation;
+ * the weights M, D (row / column sums or norms) stay FP64 and scale each row / column in FP64. */
+int fair_sirt_f32(fair_csr *A, int32_t method, const float *b, int32_t maxits, const float *x0,
+                  const fair_opts *opts, float *xout);
+int fair_sirt_f32_dev(fair_csr *A, int32_t method, const float *d_b, int32_t maxits, float *d_x,
+                      const fair_opts *opts);
+/* Batched sirt: k right-hand sides sharing A (k slices of one geometry), FP32 as above.
+ * d_B: m x k, d_X: n x k (start in, result out), row-major.  use_graph != 0 captures the whole
+ * run into a HIP graph on first use and replays it (the graph is keyed by A, d_B, d_X, method,
+ * k, maxits; the library synchronises before returning).  No counterpart in the reference:
+ * its to-do "blocking support" (README.md:6). */
+int fair_sirt_batch_dev(fair_csr *A, int32_t method, const float *d_B, int32_t k, int32_t maxits,
+                        float *d_X, const fair_opts *opts, int32_t use_graph);
+/* Autotuner of the batched SpMM: times 8 / 16 / 32 right-hand sides per wavefront for A*X and
+ * A^T*R on this matrix with k right-hand sides and keeps the fastest in the handle (kc_out[2]
+ * receives the choice, may be NULL). */
+int fair_autotune_batch(fair_csr *A, int32_t k, int32_t *kc_out);
+/* Host-array form of fair_sirt_batch_dev (B: m x k, X0 may be NULL, Xout: n x k). */
+int fair_sirt_batch(fair_csr *A, int32_t method, const float *B, int32_t k, int32_t maxits,
+                    const float *X0, const fair_opts *opts, int32_t use_graph, float *Xout);
+/* Per right-hand side ||X_j - Xref_j|| / ||Xref_j|| with FP64 sums (X FP32, Xref FP64, n x k). */
+int fair_batch_relerr_dev(const float *d_X, const double *d_Xref, int32_t n, int32_t k, double *err);
 /* normest, AIRMs.f90:1129-1173, started from v0 (the reference draws it with random_number). */
 int fair_normest(fair_csr *A, const double *v0, double tol, int32_t maxit, double *sigma);
 /* gradient_descent, AIRMs.f90:860-995.  sigma is normest's result (before squaring); iters
