@@ -77,6 +77,16 @@ SIGNATURES = {
     "fair_art": [_P, _I32, _P, _I32, _P, _P, _P, _I32, C.POINTER(Opts), _P, _I64, _P],
     "fair_art_uniform_count": [_P, _I32, _I32, _P, _P, _I32, C.POINTER(Opts), C.POINTER(_I64)],
     "fair_art_steps_dev": [_P, _P, _I64, _P, _P, C.POINTER(Opts), _P],
+    "fair_mf_create": [C.POINTER(Geom), _P, C.POINTER(_P)],
+    "fair_mf_free": [_P],
+    "fair_mf_forward": [_P, _P, _P],
+    "fair_mf_adjoint": [_P, _P, _P],
+    "fair_mf_forward_dev": [_P, _P, _P],
+    "fair_mf_adjoint_dev": [_P, _P, _P],
+    "fair_mf_segments": [_P, C.POINTER(_I64)],
+    "fair_mf_residual_dev": [_P, _P, _P, _P],
+    "fair_mf_normest": [_P, _P, _D, _I32, _P],
+    "fair_mf_gd_dev": [_P, _P, _I32, _D, _D, _D, C.POINTER(Opts), _P, _P],
 }
 _RESTYPES = {"fair_last_error": C.c_char_p, "fair_version": C.c_char_p}
 
@@ -92,6 +102,14 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         if _lib is None:
             if not os.path.exists(path):
                 raise FairError(FAIR_ERR_HIP, f"{path} is not built (run __graft_entry__.build())")
+            # torch ships its own HIP runtime under a file name that does not match the soname
+            # libamdhip64.so.7.  Loaded first, it satisfies this library's dependency and the
+            # process has one runtime; loaded after /opt/rocm's, a second HSA runtime starts
+            # and finds no device.  So torch, when present, is imported before the library.
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             lib = C.CDLL(path)
             for name, args in SIGNATURES.items():
                 f = getattr(lib, name)

@@ -165,6 +165,30 @@ int fair_art_uniform_count(fair_csr *A, int32_t method, int32_t nk, const int32_
 int fair_art_steps_dev(fair_csr *A, const int32_t *d_rows, int64_t nsteps, const double *d_b,
                        const double *d_nrA, const fair_opts *opts, double *d_x);
 
+/* ---- matrix-free fan-beam projector (FP32) ------------------------------------------------
+ * The reference's to-do "make matrix-free implementations" (README.md:7): the operator of
+ * fanlineartomo (system_generation.f90:79-307) with the same angle and ray tables, its
+ * intersections recomputed on every product instead of stored, so that problems whose CSR
+ * would exceed int32 nnz (or HBM) can be solved.  Image n*n (column-major pixel order of the
+ * reference), sinogram nth*p (angle-major). */
+typedef struct fair_mf fair_mf;
+int fair_mf_create(const fair_geom *g, const double *theta, fair_mf **P);
+int fair_mf_free(fair_mf *P);
+/* y = A x (matvecmul, sparse_matrices.f90:593-620) and x = A^T y (transvecmul, :622-656). */
+int fair_mf_forward(fair_mf *P, const float *x, float *y);
+int fair_mf_adjoint(fair_mf *P, const float *y, float *x);
+int fair_mf_forward_dev(fair_mf *P, const float *d_x, float *d_y);
+int fair_mf_adjoint_dev(fair_mf *P, const float *d_y, float *d_x);
+/* Ray-pixel segments of positive length in one product (the operator's nonzeros). */
+int fair_mf_segments(fair_mf *P, int64_t *count);
+/* r = A x - b. */
+int fair_mf_residual_dev(fair_mf *P, const float *d_x, const float *d_b, float *d_r);
+/* normest, AIRMs.f90:1129-1173, on the operator (v0: n*n host start vector). */
+int fair_mf_normest(fair_mf *P, const float *v0, double tol, int32_t maxit, double *sigma);
+/* gradient_descent, AIRMs.f90:860-995, on the operator; d_x: x0 in, result out. */
+int fair_mf_gd_dev(fair_mf *P, const float *d_b, int32_t maxiter, double rho, double tol, double sigma,
+                   const fair_opts *opts, float *d_x, int32_t *iters);
+
 #ifdef __cplusplus
 }
 #endif
