@@ -47,13 +47,14 @@ def measured_traffic() -> int | None:
     return None
 
 
-def cpu_baseline(iters: int) -> dict | None:
+def cpu_baseline(iters: int, method: str = "sart") -> dict | None:
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--iters", str(iters), "--threads", str(cores)]
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--iters", str(iters), "--threads", str(cores),
+           "--method", method]
     log("cpu baseline:", " ".join(cmd))
     import resource
 
@@ -93,9 +94,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
-    ap.add_argument("--mode", default="sart", choices=["sart", "precision", "batch"],
+    ap.add_argument("--mode", default="sart", choices=["sart", "precision", "batch", "matfree"],
                     help="sart: the headline N=512 FP64 SART line (default); precision: FP32 vs FP64 "
-                         "time to equal error at N=512; batch: 256 slices of 256^2 as one batched FP32 run")
+                         "time to equal error at N=512; batch: 256 slices of 256^2 as one batched FP32 run; "
+                         "matfree: matrix-free FP32 gradient descent at 2048^2 and at N=512 against the "
+                         "FP64 CSR method")
+    ap.add_argument("--mf-iters", type=int, default=100)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB fill and a 1 GiB sum (reference kernels for HBM counters)")
     a = ap.parse_args()
@@ -118,6 +122,8 @@ def main() -> int:
         return bench_precision(a, np, torch, _lib, airms, cf, sg, L, stream)
     if a.mode == "batch":
         return bench_batch(a, np, torch, _lib, airms, cf, sg, L, stream)
+    if a.mode == "matfree":
+        return bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream)
 
     g = cf.C2
     dat = sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta())
@@ -328,6 +334,111 @@ def bench_batch(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
                               "all_within_1e-5": bool((err <= 1e-5).all())},
            "achieved_tflops_fp32": round(flops / t_batch / 1e12, 3),
            "autotuned_rhs_per_wavefront": [kc[0], kc[1]]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
+    """Matrix-free FP32 projector: 2048^2 Shepp-Logan + seeded Gaussian blobs, fan beam 2048 angles
+    x 2900 rays (about 1.0e10 ray-pixel segments per product; the CSR would exceed int32 nnz):
+    forward / adjoint timings, normest, gradient descent (box [0,1], fixed iteration count), the
+    dot-product check; then at N=512 (720 x 725 rays) gradient descent against the plain FP64 CSR
+    method from the same sigma, for time to equal error."""
+    from fortran_airtools_amd.matrix_free import FanProjector
+    dev = torch.device("cuda:0")
+    iters = a.mf_iters
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    g = cf.C4
+    P = FanProjector(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta()))
+    nseg = P.segments()
+    xt = torch.from_numpy(cf.blobs(g.n).astype(np.float32)).to(dev)
+    b = torch.empty(P.m, dtype=torch.float32, device=dev)
+    P.matvecmul_dev(xt.data_ptr(), b.data_ptr())
+    tmp_x = torch.empty(P.n, dtype=torch.float32, device=dev)
+    tmp_y = torch.empty(P.m, dtype=torch.float32, device=dev)
+    e0, e1 = _events(torch, stream)
+    times = {}
+    for name, f in (("forward", lambda: P.matvecmul_dev(xt.data_ptr(), tmp_y.data_ptr())),
+                    ("adjoint", lambda: P.transvecmul_dev(b.data_ptr(), tmp_x.data_ptr()))):
+        f()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(5):
+            f()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1) / 5 / 1e3
+    log(f"2048^2: {nseg:.4g} segments/product; forward {times['forward'] * 1e3:.2f} ms, "
+        f"adjoint {times['adjoint'] * 1e3:.2f} ms")
+    # dot-product check <A x, y> = <x, A^T y> on two uniform pairs
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4)
+    dots = []
+    for _ in range(2):
+        u = torch.rand(P.n, generator=gen, device=dev)
+        v = torch.rand(P.m, generator=gen, device=dev)
+        P.matvecmul_dev(u.data_ptr(), tmp_y.data_ptr())
+        P.transvecmul_dev(v.data_ptr(), tmp_x.data_ptr())
+        lhs = float(torch.dot(tmp_y.double(), v.double()))
+        rhs = float(torch.dot(u.double(), tmp_x.double()))
+        dots.append(abs(lhs - rhs) / abs(lhs))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sigma = P.normest(v0=np.random.default_rng(4).random(P.n))
+    t_norm = time.perf_counter() - t0
+    x = torch.zeros(P.n, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = P.gradient_descent_dev(b.data_ptr(), x.data_ptr(), max_iter=iters, tolerance=0.0, sigma=sigma, options=o)
+    torch.cuda.synchronize()
+    t_gd = time.perf_counter() - t0
+    err_truth = float(torch.linalg.norm(x.double() - xt.double()) / torch.linalg.norm(xt.double()))
+    log(f"normest {t_norm:.3f} s (sigma {sigma:.6g}); GD {its} iterations {t_gd:.3f} s, "
+        f"rel. error to phantom {err_truth:.3e}")
+    del tmp_x, tmp_y, x, b, xt
+    P.close()
+    # N=512: FP64 CSR gradient descent against matrix-free FP32 from the same sigma
+    g2 = cf.C2
+    d2 = sg.Data(n=g2.n, p=g2.p, nth=g2.nth, r=g2.r, dw=g2.dw, sd=g2.sd, theta=g2.theta())
+    A = sg.fanlineartomo(d2)
+    xt2 = cf.grains(g2.n, 200, 2)
+    b2 = A.matvecmul(xt2)
+    sigma2 = airms.normest(A, v0=np.random.default_rng(2).random(A.n))
+    airms.gradient_descent(A, b2, 1, tolerance=0.0, options=o, sigma=sigma2)      # warm-up: transpose
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x64 = airms.gradient_descent(A, b2, iters, tolerance=0.0, options=o, sigma=sigma2)
+    t64 = time.perf_counter() - t0
+    P2 = FanProjector(d2)
+    db2 = torch.from_numpy(b2.astype(np.float32)).to(dev)
+    x32 = torch.zeros(A.n, dtype=torch.float32, device=dev)
+    P2.gradient_descent_dev(db2.data_ptr(), x32.data_ptr(), max_iter=1, tolerance=0.0, sigma=sigma2, options=o)
+    x32.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P2.gradient_descent_dev(db2.data_ptr(), x32.data_ptr(), max_iter=iters, tolerance=0.0, sigma=sigma2,
+                            options=o)
+    torch.cuda.synchronize()
+    t32 = time.perf_counter() - t0
+    xh = x32.cpu().numpy().astype(np.float64)
+    err = float(np.linalg.norm(xh - x64) / np.linalg.norm(x64))
+    log(f"N=512 GD {iters} it: FP64 CSR {t64:.3f} s, matrix-free FP32 {t32:.3f} s, rel. error {err:.3e}")
+    cpu = None if a.no_cpu else cpu_baseline(a.cpu_iters, "gd")
+    out = {"metric": METRIC, "mode": "matfree", "unit": "GD iter/s", "dtype": "f32 (fp64 ray geometry and norms)",
+           "value": round(its / t_gd, 3),
+           "config": {"workload": "2048x2048 Shepp-Logan + Gaussian blobs, fan beam 2048 x 2900 rays, matrix-free "
+                                  "gradient descent, box [0,1]", "iterations": its,
+                      "segments_per_product": nseg},
+           "forward_ms": round(times["forward"] * 1e3, 3), "adjoint_ms": round(times["adjoint"] * 1e3, 3),
+           "segments_per_s": {"forward": round(nseg / times["forward"], 1),
+                              "adjoint": round(nseg / times["adjoint"], 1),
+                              "gd": round(2 * nseg * its / t_gd, 1)},
+           "normest_s": round(t_norm, 4), "sigma": sigma, "gd_s": round(t_gd, 4),
+           "relerr_to_phantom": err_truth, "dot_product_relerr": max(dots),
+           "time_to_equal_error_n512": {"iterations": iters, "fp64_csr_s": round(t64, 4),
+                                        "matrix_free_fp32_s": round(t32, 4), "relerr": err,
+                                        "within_1e-5": err <= 1e-5},
+           "cpu_baseline_n512_gd": cpu}
     print(json.dumps(out), flush=True)
     return 0
 

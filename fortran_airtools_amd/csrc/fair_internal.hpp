@@ -175,6 +175,31 @@ int64_t exclusive_scan_i32(const int32_t *d_counts, int64_t m, int32_t *d_out);
 int lds_per_block();
 void row_norms(const Csr &A, int p, double *out);  // (sum |a|^p)^(1/p), pow-based
 
+// Matrix-free projector support in raytrace.hip: the rays whose entries must come from the
+// matrix build's own point rule (near-axis directions), with their in-range entries as a small
+// CSR table (offs: nrays + 1 offsets; cols 0-based; vals FP32 lengths).
+void mf_exact_rays(int32_t n, int32_t p, int32_t nth, const double *d_ang, const double *d_rayc,
+                   std::vector<int32_t> &rays, std::vector<int32_t> &offs, std::vector<int32_t> &cols,
+                   std::vector<float> &vals);
+
+// ---- fixed-point sums (matrix-free adjoint) --------------------------------------------------
+// Terms r * seg are scaled by S = 2^(ecap - E), where max |r| < 2^E is read from the bit
+// pattern of max |r| (*maxbits), rounded to integers and summed in 64-bit two's complement:
+// exact, so independent of the order of the adds.  ecap keeps one term below 2^51 and the
+// whole sum below 2^62.
+__device__ __forceinline__ double fix_scale(const unsigned *maxbits, int ecap) {
+    const unsigned mb = *maxbits;
+    const int E = (int)((mb >> 23) & 0xffu) - 126;   // M < 2^E (M normal or subnormal)
+    return ldexp(1.0, ecap - E);
+}
+
+// round(v) as a two's-complement 64-bit integer, |v| < 2^51: adding 1.5 * 2^52 leaves the
+// rounded integer in the low mantissa bits.
+__device__ __forceinline__ unsigned long long fix_round(double v) {
+    const double magic = 6755399441055744.0;
+    return (unsigned long long)(__double_as_longlong(v + magic) - __double_as_longlong(magic));
+}
+
 // ---- wavefront reduction --------------------------------------------------------------------
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {

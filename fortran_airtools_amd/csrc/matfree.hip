@@ -12,6 +12,7 @@
 // segment lengths used in the products and the accumulation are FP32.  The adjoint walks the
 // same segments and scatters r_i * length into the image with FP32 atomic adds, so it is the
 // exact transpose of the forward walk (the dot-product test checks it).
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -28,8 +29,8 @@
 // Both products walk rays through the pixel grid (Siddon / incremental traversal): a ray is
 // clipped to a box of pixels, then every step moves to the nearer of the next x-line and the next
 // y-line crossing.  Crossing parameters come straight from the line index in FP64,
-// t = (L - n/2 - x0) * (1/a), never accumulated, so long rays do not drift, and a walk clipped to
-// a sub-box visits exactly the segments of the full walk that lie in it.  Pixel values, segment
+// t = ((L - n/2) - x0) * (1/a), never accumulated, so long rays do not drift, and a walk clipped
+// to a sub-box visits exactly the segments of the full walk that lie in it.  Pixel values, segment
 // lengths and sums are FP32.
 //
 //  forward  one thread per ray over the whole image; MF_BATCH segments are generated before
@@ -38,9 +39,11 @@
 //  adjoint  one workgroup per (TS x TS image tile, block of MF_AB angles): for each angle the
 //           rays that can cross the tile form a contiguous range of each monotone run of
 //           tan(omega) (found by binary search); their segments inside the tile are accumulated
-//           into the tile in LDS, and the tile is added to the image once with coalesced atomics.
-//           The adjoint is therefore the transpose of the forward up to summation order (the
-//           dot-product test checks it).
+//           into the tile in LDS as 64-bit fixed-point integers, and the tile is added to a
+//           fixed-point image once with coalesced integer atomics.  The segment lengths are the
+//           forward walk's, so the adjoint is the transpose of the forward up to the rounding of
+//           the sums (the dot-product test checks it), and it is deterministic.
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -59,6 +62,18 @@ struct fair_mf {
     fair::DBuf<double> tanw;   // per ray tan omega
     int32_t nruns = 0;
     int32_t runs[2 * fair::MF_MAX_RUNS] = {};   // [begin, end) of the increasing runs of tan omega
+    // adjoint workspace: 64-bit fixed-point image (kept zero between calls) and max |r| bits;
+    // calls on one handle must not overlap
+    fair::DBuf<unsigned long long> xacc;
+    fair::DBuf<unsigned> rmax;
+    int32_t ecap = 0;   // scale exponent bound of the fixed-point sums
+    // rays within ~1e-13 of a grid line: their entries as the matrix build computes them
+    // (a small CSR: ray ids, offsets, 0-based columns, FP32 lengths), applied by table
+    fair::DBuf<int32_t> ex_rays, ex_offs, ex_cols;
+    fair::DBuf<float> ex_vals;
+    int32_t n_exact = 0;
+    int64_t ex_segments = 0;    // table entries of positive length
+    fair::DBuf<uint8_t> flag;   // per ray: 1 = in the table (skipped by the traversal kernels)
 };
 
 namespace fair {
@@ -67,10 +82,14 @@ namespace {
 constexpr int MF_BATCH = 8;   // segments per gather batch (forward)
 constexpr int MF_TS = 64;     // adjoint image tile side
 constexpr int MF_AB = 64;     // adjoint angles per workgroup
+// LDS row stride of the adjoint tile: an odd stride puts x-neighbours (rays of one angle that
+// run mostly along y hit them in the same step) in different banks.
+constexpr int MF_LDS_STRIDE = MF_TS + 1;
 
 struct MfArgs {
     int32_t n, p, nth, nruns;
     const double *ang, *rayc, *tanw;
+    const uint8_t *flag;
     int32_t runs[2 * MF_MAX_RUNS];
 };
 
@@ -90,19 +109,26 @@ __device__ __forceinline__ void ray_line(const MfArgs &g, int i, int j, double &
 // indices: ix = floor(x + n/2), iy = floor(y + n/2)).  A line with a == 0 belongs to the column
 // floor(x0 + n/2) (half-open pixels), likewise b == 0.
 struct Walk {
-    double x0, y0, inva, invb, h, len, t, t1, tx, ty;
+    double x0, y0, h, inva, invb, len, t, t1, tx, ty;
     int ix, iy, sx, sy, ixe, iye;
     bool live;
 
-    __device__ __forceinline__ double tX(int L) const { return ((double)L - h - x0) * inva; }
-    __device__ __forceinline__ double tY(int L) const { return ((double)L - h - y0) * invb; }
+    // t of the grid line with index L: ((L - n/2) - x0) * (1/a).  L - n/2 is exact, so a line
+    // through the ray's own start coordinate gets t = 0 exactly, as the reference's
+    // (x - x0th) / a (system_generation.f90:160) does; a fused L * (1/a) + c form cancels
+    // catastrophically for nearly axis-aligned rays (a ~ 1e-16 at theta = pi).
+    __device__ __forceinline__ double tline(int L, double p0, double inv) const {
+        return (((double)L - h) - p0) * inv;
+    }
+    __device__ __forceinline__ double tX(int L) const { return tline(L, x0, inva); }
+    __device__ __forceinline__ double tY(int L) const { return tline(L, y0, invb); }
 
     __device__ __forceinline__ void init(double px0, double py0, double a, double b, int n, int bx0, int bx1,
                                          int by0, int by1) {
         live = false;
+        h = 0.5 * (double)n;
         x0 = px0;
         y0 = py0;
-        h = 0.5 * (double)n;
         len = sqrt(a * a + b * b);
         if (!(len > 0.0)) return;
         inva = a != 0.0 ? 1.0 / a : 0.0;
@@ -147,31 +173,29 @@ struct Walk {
     }
 
     // The next segment (pixel ix, iy; length seg >= 0).  Call only while live; every call moves
-    // ix or iy one pixel towards the box edge or ends the walk.
+    // ix or iy one pixel towards the box edge or ends the walk.  Pieces of length <= 1e-10 count
+    // as 0, as the matrix build drops them (system_generation.f90:196).
     __device__ __forceinline__ void next(int &oix, int &oiy, float &seg) {
         const double tn = fmin(fmin(tx, ty), t1);
-        seg = (float)(fmax(tn - t, 0.0) * len);
+        const double d = (tn - t) * len;
+        seg = d > 1e-10 ? (float)d : 0.f;
         oix = ix;
         oiy = iy;
         t = tn;
-        if (tn >= t1) {
-            live = false;
-        } else if (tx <= ty) {
-            ix += sx;
-            if (ix == ixe) live = false;
-            else tx = tX(ix + (sx > 0));
-        } else {
-            iy += sy;
-            if (iy == iye) live = false;
-            else ty = tY(iy + (sy > 0));
-        }
+        const bool stepx = tx <= ty;
+        ix += stepx ? sx : 0;
+        iy += stepx ? 0 : sy;
+        live = tn < t1 && ix != ixe && iy != iye;
+        const double tnew = tline(stepx ? ix + (sx > 0) : iy + (sy > 0), stepx ? x0 : y0, stepx ? inva : invb);
+        tx = stepx ? tnew : tx;
+        ty = stepx ? ty : tnew;
     }
 };
 
 __global__ void __launch_bounds__(256) k_mf_forward(MfArgs g, int64_t m, const float *__restrict__ x,
                                                     const float *__restrict__ b, float *__restrict__ y) {
     const int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ray >= m) return;
+    if (ray >= m || g.flag[ray]) return;
     const int i = (int)(ray / g.p), j = (int)(ray - (int64_t)i * g.p);
     double x0, y0, a, bb;
     ray_line(g, i, j, x0, y0, a, bb);
@@ -193,7 +217,7 @@ __global__ void __launch_bounds__(256) k_mf_forward(MfArgs g, int64_t m, const f
             }
         }
 #pragma unroll
-        for (int k = 0; k < MF_BATCH; ++k) acc += sg[k] * x[pix[k]];
+        for (int k = 0; k < MF_BATCH; ++k) acc = fmaf(sg[k], x[pix[k]], acc);
     }
     y[ray] = b ? acc - b[ray] : acc;
 }
@@ -202,7 +226,7 @@ __global__ void __launch_bounds__(256) k_mf_forward(MfArgs g, int64_t m, const f
 __global__ void __launch_bounds__(256) k_mf_count(MfArgs g, int64_t m, unsigned long long *out) {
     const int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long c = 0;
-    if (ray < m) {
+    if (ray < m && !g.flag[ray]) {
         const int i = (int)(ray / g.p), j = (int)(ray - (int64_t)i * g.p);
         double x0, y0, a, bb;
         ray_line(g, i, j, x0, y0, a, bb);
@@ -237,17 +261,47 @@ __device__ __forceinline__ int upper_bound_d(const double *v, int lo, int hi, do
     return lo;
 }
 
+// Fixed-point accumulation of the adjoint (fix_scale / fix_round): LDS float atomics are slow on
+// gfx950 (tens of times slower than integer ones), so every term r_i * seg is scaled by a power
+// of two, rounded to an integer and summed in 64-bit integers; the rounding error of a term is at
+// most 2^-(ecap+1) * max|r|, far below FP32 resolution, and the sums do not depend on order.
+__global__ void __launch_bounds__(256) k_mf_absmax(const float *__restrict__ r, int64_t m, unsigned *maxbits) {
+    unsigned v = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256)
+        v = max(v, __float_as_uint(r[i]) & 0x7fffffffu);   // |r| bits order like the values
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, v);
+}
+
+__global__ void k_mf_reset(unsigned *maxbits) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *maxbits = 0u;
+}
+
+// x = acc / S (FP32), and acc is cleared for the next call.
+__global__ void __launch_bounds__(256) k_mf_fixed_to_f32(unsigned long long *__restrict__ acc, int64_t nn,
+                                                         const unsigned *maxbits, int ecap, float *__restrict__ x) {
+    const double inv = 1.0 / fix_scale(maxbits, ecap);
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nn; i += (int64_t)gridDim.x * 256) {
+        const long long v = (long long)acc[i];
+        x[i] = (float)((double)v * inv);
+        acc[i] = 0ull;
+    }
+}
+
 // Workgroup (tile, angle block); blockIdx = angle block * tiles + tile.
 __global__ void __launch_bounds__(256) k_mf_adjoint_tiles(MfArgs g, int32_t nty, int32_t ntiles,
-                                                          const float *__restrict__ r, float *__restrict__ x) {
-    __shared__ float acc[MF_TS * MF_TS];
+                                                          const float *__restrict__ r, const unsigned *maxbits,
+                                                          int32_t ecap, unsigned long long *__restrict__ xacc) {
+    __shared__ unsigned long long acc[MF_TS * MF_LDS_STRIDE];
     __shared__ int e_lo[256], e_pref[257];
     const int n = g.n, tid = threadIdx.x;
     const int tile = blockIdx.x % ntiles, abk = blockIdx.x / ntiles;
     const int X0 = (tile / nty) * MF_TS, Y0 = (tile % nty) * MF_TS;
     const int X1 = min(X0 + MF_TS, n), Y1 = min(Y0 + MF_TS, n);
     const int i0 = abk * MF_AB, na = min(MF_AB, g.nth - i0);
-    for (int q = tid; q < MF_TS * MF_TS; q += 256) acc[q] = 0.f;
+    const double S = fix_scale(maxbits, ecap);
+    for (int q = tid; q < MF_TS * MF_LDS_STRIDE; q += 256) acc[q] = 0ull;
     // candidate rays: entry e = (angle, run); host guarantees MF_AB * nruns <= 256
     const int E = na * g.nruns;
     int cnt = 0;
@@ -300,8 +354,10 @@ __global__ void __launch_bounds__(256) k_mf_adjoint_tiles(MfArgs g, int32_t nty,
         }
         const int e = lo;
         const int i = i0 + e / g.nruns, j = e_lo[e] + (wi - e_pref[e]);
-        const float rv = r[(int64_t)i * g.p + j];
-        if (rv == 0.f) continue;
+        const int64_t ray = (int64_t)i * g.p + j;
+        const float rv = r[ray];
+        if (rv == 0.f || g.flag[ray]) continue;
+        const double rs = (double)rv * S;
         double x0, y0, a, bb;
         ray_line(g, i, j, x0, y0, a, bb);
         Walk w;
@@ -310,22 +366,17 @@ __global__ void __launch_bounds__(256) k_mf_adjoint_tiles(MfArgs g, int32_t nty,
             int ix, iy;
             float sg;
             w.next(ix, iy, sg);
-            atomicAdd(&acc[(ix - X0) * MF_TS + (iy - Y0)], rv * sg);
+            atomicAdd(&acc[(ix - X0) * MF_LDS_STRIDE + (iy - Y0)], fix_round(rs * (double)sg));
         }
     }
     __syncthreads();
     for (int q = tid; q < MF_TS * MF_TS; q += 256) {
         const int lx = q / MF_TS, ly = q - lx * MF_TS;
         if (X0 + lx < X1 && Y0 + ly < Y1) {
-            const float v = acc[q];
-            if (v != 0.f) unsafeAtomicAdd(&x[(int64_t)(X0 + lx) * n + (n - 1 - (Y0 + ly))], v);
+            const unsigned long long v = acc[lx * MF_LDS_STRIDE + ly];
+            if (v != 0ull) atomicAdd(&xacc[(int64_t)(X0 + lx) * n + (n - 1 - (Y0 + ly))], v);
         }
     }
-}
-
-__global__ void k_zero_f32(float *x, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = 0.f;
 }
 
 // Sum-of-products reductions into one FP64 accumulator (block partials, one atomic per wavefront).
@@ -381,8 +432,32 @@ __global__ void k_neg_f32(const float *b, float *r, int64_t n) {
 
 constexpr int RED_GRID = 1024;
 
+// Table rays: forward sums in emission order (FP32), adjoint adds fixed-point terms.
+__global__ void __launch_bounds__(64) k_mf_table_forward(const int32_t *rays, const int32_t *offs,
+                                                         const int32_t *cols, const float *vals, int32_t nr,
+                                                         const float *x, const float *b, float *y) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nr) return;
+    float acc = 0.f;
+    for (int k = offs[q]; k < offs[q + 1]; ++k) acc = fmaf(vals[k], x[cols[k]], acc);
+    const int32_t ray = rays[q];
+    y[ray] = b ? acc - b[ray] : acc;
+}
+
+__global__ void __launch_bounds__(64) k_mf_table_adjoint(const int32_t *rays, const int32_t *offs,
+                                                         const int32_t *cols, const float *vals, int32_t nr,
+                                                         const float *r, const unsigned *maxbits, int32_t ecap,
+                                                         unsigned long long *xacc) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nr) return;
+    const float rv = r[rays[q]];
+    if (rv == 0.f) return;
+    const double rs = (double)rv * fix_scale(maxbits, ecap);
+    for (int k = offs[q]; k < offs[q + 1]; ++k) atomicAdd(&xacc[cols[k]], fix_round(rs * (double)vals[k]));
+}
+
 MfArgs args(const fair_mf *h) {
-    MfArgs a{h->n, h->p, h->nth, h->nruns, h->ang.p, h->rayc.p, h->tanw.p, {}};
+    MfArgs a{h->n, h->p, h->nth, h->nruns, h->ang.p, h->rayc.p, h->tanw.p, h->flag.p, {}};
     for (int q = 0; q < 2 * MF_MAX_RUNS; ++q) a.runs[q] = h->runs[q];
     return a;
 }
@@ -390,18 +465,29 @@ MfArgs args(const fair_mf *h) {
 void mf_forward(const fair_mf *h, const float *x, const float *b, float *y) {
     if (h->m > 0)
         hipLaunchKernelGGL(k_mf_forward, dim3(grid_for(h->m, 256)), dim3(256), 0, stream(), args(h), h->m, x, b, y);
+    if (h->n_exact > 0)
+        hipLaunchKernelGGL(k_mf_table_forward, dim3(grid_for(h->n_exact, 64)), dim3(64), 0, stream(), h->ex_rays.p,
+                           h->ex_offs.p, h->ex_cols.p, h->ex_vals.p, h->n_exact, x, b, y);
     launch_check("matrix-free forward");
 }
 
 void mf_adjoint(const fair_mf *h, const float *r, float *x) {
     const int64_t n = (int64_t)h->n * h->n;
-    hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(n, 256)), dim3(256), 0, stream(), x, n);
     const int nt = (h->n + MF_TS - 1) / MF_TS, nab = (h->nth + MF_AB - 1) / MF_AB;
     const int64_t blocks = (int64_t)nt * nt * nab;
     FAIR_CHECK(blocks < INT32_MAX, FAIR_ERR_OVERFLOW, "matrix-free adjoint grid too large");
-    if (h->m > 0)
+    hipLaunchKernelGGL(k_mf_reset, dim3(1), dim3(64), 0, stream(), h->rmax.p);
+    if (h->m > 0) {
+        hipLaunchKernelGGL(k_mf_absmax, dim3(1024), dim3(256), 0, stream(), r, h->m, h->rmax.p);
         hipLaunchKernelGGL(k_mf_adjoint_tiles, dim3((unsigned)blocks), dim3(256), 0, stream(), args(h), nt, nt * nt,
-                           r, x);
+                           r, (const unsigned *)h->rmax.p, h->ecap, h->xacc.p);
+        if (h->n_exact > 0)
+            hipLaunchKernelGGL(k_mf_table_adjoint, dim3(grid_for(h->n_exact, 64)), dim3(64), 0, stream(),
+                               h->ex_rays.p, h->ex_offs.p, h->ex_cols.p, h->ex_vals.p, h->n_exact, r,
+                               (const unsigned *)h->rmax.p, h->ecap, h->xacc.p);
+    }
+    hipLaunchKernelGGL(k_mf_fixed_to_f32, dim3(1024), dim3(256), 0, stream(), h->xacc.p, n,
+                       (const unsigned *)h->rmax.p, h->ecap, x);
     launch_check("matrix-free adjoint");
 }
 
@@ -490,6 +576,33 @@ extern "C" int fair_mf_create(const fair_geom *g, const double *theta, fair_mf *
         h->ang.upload(ang.data(), ang.size());
         h->rayc.upload(rayc.data(), rayc.size());
         h->tanw.upload(tanw.data(), tanw.size());
+        // fixed-point range: one term below 2^49.5, all p*nth*sqrt(2)*M*S below 2^62
+        const int ebits = (int)std::ceil(std::log2(1.5 * (double)p * (double)nth));
+        h->ecap = std::min(49, 62 - ebits);
+        FAIR_CHECK(h->ecap >= 24, FAIR_ERR_OVERFLOW, "too many rays for the fixed-point adjoint");
+        const int64_t nn = (int64_t)n * n;
+        h->xacc.alloc(nn);
+        h->rmax.alloc(1);
+        FAIR_HIP(hipMemsetAsync(h->xacc.p, 0, sizeof(unsigned long long) * nn, stream()));
+        FAIR_HIP(hipMemsetAsync(h->rmax.p, 0, sizeof(unsigned), stream()));
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        std::vector<int32_t> ex, eoff, ecol;
+        std::vector<float> eval;
+        mf_exact_rays(n, p, nth, h->ang.p, h->rayc.p, ex, eoff, ecol, eval);
+        std::vector<uint8_t> flag((size_t)h->m, 0);
+        for (int32_t r : ex) flag[r] = 1;
+        h->n_exact = (int32_t)ex.size();
+        for (float v : eval) h->ex_segments += v > 0.f;
+        h->ex_rays.alloc(std::max<size_t>(ex.size(), 1));
+        h->ex_offs.alloc(eoff.size());
+        h->ex_cols.alloc(std::max<size_t>(ecol.size(), 1));
+        h->ex_vals.alloc(std::max<size_t>(eval.size(), 1));
+        h->ex_rays.upload(ex.data(), ex.size());
+        h->ex_offs.upload(eoff.data(), eoff.size());
+        h->ex_cols.upload(ecol.data(), ecol.size());
+        h->ex_vals.upload(eval.data(), eval.size());
+        h->flag.alloc((size_t)h->m);
+        h->flag.upload(flag.data(), flag.size());
         FAIR_HIP(hipStreamSynchronize(stream()));
         *out = h.release();
     });
@@ -552,7 +665,7 @@ extern "C" int fair_mf_segments(fair_mf *h, int64_t *count) {
         unsigned long long v = 0;
         c.download(&v, 1);
         FAIR_HIP(hipStreamSynchronize(stream()));
-        *count = (int64_t)v;
+        *count = (int64_t)v + h->ex_segments;
     });
 }
 

@@ -16,6 +16,7 @@
 // Bit-exactness: host-side trig (cos, sin, atan from the C library, as the reference calls
 // them) is uploaded as tables; the device uses only IEEE +, -, *, /, sqrt and floor, and this
 // file is compiled with -ffp-contract=off so no product is fused into an add.
+#include <algorithm>
 #include <cmath>
 
 #include "fair_internal.hpp"
@@ -70,10 +71,18 @@ __device__ __forceinline__ RayGeom ray_geom(const RayArgs &g, int64_t ray) {
 // (the parallel-beam linspace of :331 gives the same exact values).
 __device__ __forceinline__ double gridc(int k, double half) { return (double)k - half - 1.0; }
 
+// Output of the fill pass: entry cnt of the ray gets the one-based column col and length tv.
 struct Emit {
     int32_t *JA;
     double *vA;
     int cap;   // entries reserved for this ray by the count pass
+
+    __device__ __forceinline__ void emit(int cnt, int col, double tv) {
+        if (cnt < cap) {
+            JA[cnt] = col - 1;
+            vA[cnt] = tv;
+        }
+    }
 };
 
 // Filter + de-duplication + emission for one point in sorted order (:183-203 / :264-297).
@@ -85,14 +94,14 @@ struct PointSink {
     int cnt = 0;
     bool bad_col = false;
 
-    template <bool FILL>
-    __device__ __forceinline__ void push(double x, double y, bool excl, Emit e) {
+    template <bool FILL, class E>
+    __device__ __forceinline__ void push(double x, double y, bool excl, E &e) {
         if (!(x >= lo && x <= hi && y >= lo && y <= hi)) return;
         if (have_prev) {
             double dx = px - x, dy = py - y;
             double tv = sqrt(dx * dx + dy * dy);
             if (tv > 1e-10 && !excl) {
-                if (FILL && cnt < e.cap) {
+                if (FILL) {
                     double xm = .5 * ((px + x) + nd);
                     double ym = .5 * ((py + y) + nd);
                     // one-based; the reference stops only on 0 (:293).  Rays that run along the
@@ -101,8 +110,7 @@ struct PointSink {
                     // they are kept bit-exact and skipped by every product kernel.
                     int col = (int)floor(xm) * n + (n - (int)floor(ym));
                     if (col == 0) bad_col = true;
-                    e.JA[cnt] = col - 1;
-                    e.vA[cnt] = tv;
+                    e.emit(cnt, col, tv);
                 }
                 cnt++;
             }
@@ -115,8 +123,8 @@ struct PointSink {
 
 // Regular rays: merge of the x-crossing and y-crossing sequences.
 // Returns -1 when the ray must take the exact path.
-template <int BEAM, bool FILL>
-__device__ int merge_ray(const RayArgs &g, int64_t ray, Emit e, bool *bad_col) {
+template <int BEAM, bool FILL, class E>
+__device__ int merge_ray(const RayArgs &g, int64_t ray, E &e, bool *bad_col) {
     RayGeom r = ray_geom<BEAM>(g, ray);
     if (!(r.a != 0.0 && r.b != 0.0)) return -1;   // axis-aligned (or NaN) direction
     const int n = g.n;
@@ -143,13 +151,13 @@ __device__ int merge_ray(const RayArgs &g, int64_t ray, Emit e, bool *bad_col) {
         if (takex) {
             double xg = gridc(kx0 + dkx * cx, half);
             double yx = r.b * tx + r.y0th;            // :164
-            s.template push<FILL>(xg, yx, r.excl, e);
+            s.template push<FILL, E>(xg, yx, r.excl, e);
             ++cx;
             if (cx <= n) tx = (gridc(kx0 + dkx * cx, half) - r.x0th) / r.a;
         } else {
             double yg = gridc(ky0 + dky * cy, half);
             double xy = r.a * ty + r.x0th;            // :169
-            s.template push<FILL>(xy, yg, r.excl, e);
+            s.template push<FILL, E>(xy, yg, r.excl, e);
             ++cy;
             if (cy <= n) ty = (gridc(ky0 + dky * cy, half) - r.y0th) / r.b;
         }
@@ -165,7 +173,8 @@ __global__ void __launch_bounds__(256) k_count(RayArgs g, int64_t m, int32_t *co
     int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ray >= m) return;
     bool bad = false;
-    int c = merge_ray<BEAM, false>(g, ray, Emit{nullptr, nullptr, 0}, &bad);
+    Emit none{nullptr, nullptr, 0};
+    int c = merge_ray<BEAM, false>(g, ray, none, &bad);
     if (c < 0) {
         irregular[ray] = 1;
         counts[ray] = 0;
@@ -185,7 +194,8 @@ __global__ void __launch_bounds__(256) k_fill(RayArgs g, int64_t m, const int32_
     if (ray >= m || irregular[ray]) return;
     int32_t off = IA[ray];
     bool bad = false;
-    int c = merge_ray<BEAM, true>(g, ray, Emit{JA + off, vA + off, IA[ray + 1] - off}, &bad);
+    Emit e{JA + off, vA + off, IA[ray + 1] - off};
+    int c = merge_ray<BEAM, true>(g, ray, e, &bad);
     if (bad) atomicOr(err, 1);
     if (c != IA[ray + 1] - off) atomicOr(err, 2);
 }
@@ -197,15 +207,11 @@ struct ExactScratch {
     int32_t *ind, *stk;    // (2n+2) and 2*(2n+2)
 };
 
-template <int BEAM, bool FILL>
-__global__ void k_exact(RayArgs g, const int32_t *irr_list, int32_t nirr, char *scratch,
-                        size_t per_ray, int32_t *counts, const int32_t *IA, int32_t *JA,
-                        double *vA, int32_t *err) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nirr) return;
-    const int64_t ray = irr_list[q];
+// The reference's algorithm for one ray; base: 36 * (2n+2) bytes of scratch.  Returns the
+// number of entries (columns of any value) the ray produces.
+template <int BEAM, bool FILL, class E>
+__device__ int exact_ray(const RayArgs &g, int64_t ray, char *base, E &e, bool *bad_col) {
     const int n = g.n, nn = 2 * n + 2;
-    char *base = scratch + per_ray * (size_t)q;
     double *t = (double *)base - 1;
     double *xx = (double *)base + nn - 1;
     double *yy = (double *)base + 2 * nn - 1;
@@ -253,18 +259,81 @@ __global__ void k_exact(RayArgs g, const int32_t *irr_list, int32_t nirr, char *
     s.hi = half;
     s.nd = (double)n;
     s.n = n;
+    for (int k = 1; k <= nn; ++k) s.template push<FILL, E>(xx[ind[k]], yy[ind[k]], r.excl, e);
+    if (s.bad_col) *bad_col = true;
+    return s.cnt;
+}
+
+// Irregular rays: the reference algorithm verbatim (t arrays, Hoare quicksort with middle
+// pivot, permutation, filter, de-duplication), one thread per ray, arrays in global scratch.
+template <int BEAM, bool FILL>
+__global__ void k_exact(RayArgs g, const int32_t *irr_list, int32_t nirr, char *scratch,
+                        size_t per_ray, int32_t *counts, const int32_t *IA, int32_t *JA,
+                        double *vA, int32_t *err) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nirr) return;
+    const int64_t ray = irr_list[q];
     Emit e{nullptr, nullptr, 0};
     if (FILL) {
         int32_t off = IA[ray];
         e = Emit{JA + off, vA + off, IA[ray + 1] - off};
     }
-    for (int k = 1; k <= nn; ++k) s.template push<FILL>(xx[ind[k]], yy[ind[k]], r.excl, e);
+    bool bad = false;
+    const int c = exact_ray<BEAM, FILL, Emit>(g, ray, scratch + per_ray * (size_t)q, e, &bad);
     if (FILL) {
-        if (s.bad_col) atomicOr(err, 1);
-        if (s.cnt != IA[ray + 1] - IA[ray]) atomicOr(err, 2);
+        if (bad) atomicOr(err, 1);
+        if (c != IA[ray + 1] - IA[ray]) atomicOr(err, 2);
     } else {
-        counts[ray] = s.cnt;
+        counts[ray] = c;
     }
+}
+
+// ---- matrix-free projector: rays that need the reference's own point rule --------------------
+// A ray within ~1e-13 of a grid line (direction nearly, but not exactly, axis-aligned, such as
+// the central ray at theta = pi) has segment midpoints that the reference's floor(xm) rule
+// (system_generation.f90:285-293) assigns by rounding, and crossings that tie exactly; an exact
+// traversal reproduces neither.  The matrix-free projector keeps the entries of such rays, as
+// the matrix build computes them, in a small table.
+constexpr double NEAR_AXIS = 1e-10;
+
+// In-range entries of one ray: 0-based column and FP32 length.
+struct MfEntries {
+    int32_t *cols;
+    float *vals;
+    int32_t nn, k;
+    __device__ __forceinline__ void emit(int, int col, double tv) {
+        const int c = col - 1;
+        if ((uint32_t)c < (uint32_t)nn) {
+            if (cols) {
+                cols[k] = c;
+                vals[k] = (float)tv;
+            }
+            ++k;
+        }
+    }
+};
+
+__global__ void __launch_bounds__(256) k_mf_near_axis(RayArgs g, int64_t m, int32_t *list, int32_t *cnt,
+                                                      int32_t cap) {
+    const int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= m) return;
+    const RayGeom r = ray_geom<BEAM_FAN>(g, ray);
+    const double fa = fabs(r.a), fb = fabs(r.b);
+    if (!(r.a != 0.0 && r.b != 0.0 && (fa < NEAR_AXIS * fb || fb < NEAR_AXIS * fa))) return;
+    const int q = atomicAdd(cnt, 1);
+    if (q < cap) list[q] = (int32_t)ray;
+}
+
+// FILL = false: per listed ray the number of in-range entries; true: write them at offs[q].
+template <bool FILL>
+__global__ void k_mf_exact_entries(RayArgs g, const int32_t *list, int32_t nlist, char *scratch, size_t per_ray,
+                                   int32_t *counts, const int32_t *offs, int32_t *cols, float *vals) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nlist) return;
+    MfEntries e{FILL ? cols + offs[q] : nullptr, FILL ? vals + offs[q] : nullptr, g.n * g.n, 0};
+    bool bad = false;
+    exact_ray<BEAM_FAN, true, MfEntries>(g, list[q], scratch + per_ray * (size_t)q, e, &bad);
+    if (!FILL) counts[q] = e.k;
 }
 
 // ---- exclusive scan of int32 counts into int32 offsets (with int64 overflow check) ---------
@@ -359,6 +428,58 @@ int64_t exclusive_scan_i32(const int32_t *d_counts, int64_t m, int32_t *d_out) {
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(256), 0, stream(), d_counts, m, part.p, d_out);
     launch_check("exclusive_scan");
     return total;
+}
+
+// Matrix-free projector support (fan beam tables as in trace<BEAM_FAN>): the near-axis rays
+// and their entries by the matrix build's rule.
+void mf_exact_rays(int32_t n, int32_t p, int32_t nth, const double *d_ang, const double *d_rayc,
+                   std::vector<int32_t> &rays, std::vector<int32_t> &offs, std::vector<int32_t> &cols,
+                   std::vector<float> &vals) {
+    const RayArgs g{n, p, nth, d_ang, d_rayc};
+    const int64_t m = (int64_t)nth * p;
+    // at most the central rays of the angles close to multiples of pi/2
+    const int32_t cap = 4 * nth + 64;
+    DBuf<int32_t> list(cap), cnt(1);
+    FAIR_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t), stream()));
+    if (m > 0)
+        hipLaunchKernelGGL(k_mf_near_axis, dim3(grid_for(m, 256)), dim3(256), 0, stream(), g, m, list.p, cnt.p, cap);
+    launch_check("near-axis ray list");
+    int32_t hc = 0;
+    cnt.download(&hc, 1);
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    FAIR_CHECK(hc <= cap, FAIR_ERR_INTERNAL, "too many near-axis rays");
+    rays.assign(hc, 0);
+    list.download(rays.data(), hc);
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    std::sort(rays.begin(), rays.end());
+    offs.assign(1, 0);
+    cols.clear();
+    vals.clear();
+    if (hc == 0) return;
+    list.upload(rays.data(), hc);
+    const size_t per_ray = 36 * (size_t)(2 * n + 2);
+    DBuf<char> scratch(per_ray * hc);
+    DBuf<int32_t> dcnt(hc), doff(hc);
+    hipLaunchKernelGGL(k_mf_exact_entries<false>, dim3(grid_for(hc, 64)), dim3(64), 0, stream(), g, list.p, hc,
+                       scratch.p, per_ray, dcnt.p, (const int32_t *)nullptr, (int32_t *)nullptr, (float *)nullptr);
+    launch_check("near-axis entry count");
+    std::vector<int32_t> c(hc);
+    dcnt.download(c.data(), hc);
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    for (int32_t q = 0; q < hc; ++q) offs.push_back(offs.back() + c[q]);
+    const int32_t tot = offs.back();
+    if (tot == 0) return;
+    doff.upload(offs.data(), hc);
+    DBuf<int32_t> dcols(tot);
+    DBuf<float> dvals(tot);
+    hipLaunchKernelGGL(k_mf_exact_entries<true>, dim3(grid_for(hc, 64)), dim3(64), 0, stream(), g, list.p, hc,
+                       scratch.p, per_ray, dcnt.p, (const int32_t *)doff.p, dcols.p, dvals.p);
+    launch_check("near-axis entries");
+    cols.assign(tot, 0);
+    vals.assign(tot, 0.f);
+    dcols.download(cols.data(), tot);
+    dvals.download(vals.data(), tot);
+    FAIR_HIP(hipStreamSynchronize(stream()));
 }
 
 namespace {

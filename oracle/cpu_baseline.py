@@ -1,12 +1,13 @@
 """CPU baseline timing of the reference path (test infrastructure; bench.py's cpu_baseline leg).
 
-Times the reference's own `sirt` (oracle/_ref, the reference's Fortran compiled with flang
+Times the reference's own `sirt` or `gradient_descent` (--method gd) (oracle/_ref, the reference's Fortran compiled with flang
 -O3 -fopenmp; kind "reference") on the host cores, or the C restatement (kind "port",
 single-threaded) when the compiled reference is absent.  The matrix comes from the C oracle's
 ray tracer (bit-identical to the reference's fanlineartomo) so that the bounded sample does not
-spend minutes in the reference's serial build.  The timed quantity is one SIRT iteration:
-(t(k iterations) - t(0 iterations)) / k, so the serial set-up (row norms, double transpose)
-cancels; k grows until the iterations take at least 2 s.
+spend minutes in the reference's serial build.  The timed quantity is one iteration:
+(t(k iterations) - t(0 iterations)) / k, so the serial set-up (row norms, double transpose;
+gradient descent's normest, started from the same seed every run) cancels; k grows until the
+iterations take at least 2 s.
 
 usage: python -m oracle.cpu_baseline --n 512 --p 725 --nth 720 --step 0.5 --iters 3 --threads 16
 Prints one JSON object.
@@ -49,6 +50,7 @@ def main() -> None:
     n = a.n * a.n
     rng = np.random.default_rng(2)
     b = o.matvec(IA, JA, vA, rng.random(n))
+    gd = a.method == "gd"
     if Reference.available():
         ref = Reference()
         kind, cores = "reference", a.threads
@@ -56,14 +58,22 @@ def main() -> None:
         def run(k):
             ref.set(0, IA, JA, vA, n)
             t = time.perf_counter()
-            ref.sirt(0, a.method, b, k, lb=0.0, ub=1.0)
+            if gd:
+                ref.rng_seed(1)          # the same normest start (and iteration count) every run
+                ref.gd(0, b, k, tol=0.0, lb=0.0, ub=1.0)
+            else:
+                ref.sirt(0, a.method, b, k, lb=0.0, ub=1.0)
             return time.perf_counter() - t
     else:
         kind, cores = "port", 1
+        sigma = o.normest(IA, JA, vA, n, rng.random(n))[0] if gd else 0.0
 
         def run(k):
             t = time.perf_counter()
-            o.sirt(a.method, IA, JA, vA, n, b, k, lb=0.0, ub=1.0)
+            if gd:
+                o.gd(IA, JA, vA, n, b, k, sigma, tol=0.0, lb=0.0, ub=1.0)
+            else:
+                o.sirt(a.method, IA, JA, vA, n, b, k, lb=0.0, ub=1.0)
             return time.perf_counter() - t
     # set-up only (maxits = 0), then grow k until the iterations dominate the timing noise
     t_setup = sorted(run(0) for _ in range(3))[1]
