@@ -162,6 +162,12 @@ def main() -> int:
     log(f"SART {a.steps} iterations: {t_sart * 1e3:.2f} ms, {it_s:.1f} it/s, "
         f"{bytes_it / (t_sart / a.steps) / 1e9:.1f} GB/s per iteration")
 
+    # the same iterations through the host-array entry point (b and x0 cross PCIe in, x out)
+    t0 = time.perf_counter()
+    airms.sirt("sart", A, b, [a.steps], x0=np.zeros(n), options=opts)
+    t_host = time.perf_counter() - t0
+    log(f"SART {a.steps} iterations from host arrays (PCIe-inclusive): {t_host * 1e3:.2f} ms")
+
     # the SpMV kernel alone (MMRS over the non-empty rows), HIP events on the launch stream
     xs = torch.rand(n, dtype=torch.float64, device=dev)
     ys = torch.zeros(m, dtype=torch.float64, device=dev)
@@ -204,6 +210,7 @@ def main() -> int:
                    "n": g.n, "angles": g.nth, "rays": g.p, "rows": m, "cols": n, "nnz": nnz,
                    "global_batch": 1, "seq_len": 0, "parallelism": "single GPU",
                    "sart_gbs_per_iteration": round(bytes_it / (t_sart / a.steps) / 1e9, 2),
+                   "pcie_inclusive_iter_per_s": round(a.steps / t_host, 3),
                    "matrix_build_s": round(t_build, 4)},
         "roofline": {"kernel": "k_spmv_blocked<double, SrcPtr<double>, EpiStore<double>>",
                      "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,

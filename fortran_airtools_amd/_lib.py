@@ -87,6 +87,7 @@ SIGNATURES = {
     "fair_mf_residual_dev": [_P, _P, _P, _P],
     "fair_mf_normest": [_P, _P, _D, _I32, _P],
     "fair_mf_gd_dev": [_P, _P, _I32, _D, _D, _D, C.POINTER(Opts), _P, _P],
+    "fair_mf_gd": [_P, _P, _I32, _P, _D, _D, _D, C.POINTER(Opts), _P, _P],
 }
 _RESTYPES = {"fair_last_error": C.c_char_p, "fair_version": C.c_char_p}
 

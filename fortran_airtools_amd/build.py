@@ -63,7 +63,7 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 def build_fortran(verbose: bool = False) -> str | None:
     """Compile the Fortran drop-in modules (fortran/*.f90, in dependency order) into
     libfair_fortran.a with their .mod files under _build/fmod."""
-    order = ["fair_capi.f90", "sparse_matrices.f90", "system_generation.f90", "airms.f90"]
+    order = ["fair_capi.f90", "sparse_matrices.f90", "system_generation.f90", "airms.f90", "fair_extensions.f90"]
     srcs = [os.path.join(FORTRAN, f) for f in order if os.path.exists(os.path.join(FORTRAN, f))]
     if not srcs:
         return None

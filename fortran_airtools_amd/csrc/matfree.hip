@@ -743,3 +743,21 @@ extern "C" int fair_mf_gd_dev(fair_mf *h, const float *d_b, int32_t maxiter, dou
         if (iters) *iters = accepted;
     });
 }
+
+// Host-array form of fair_mf_gd_dev: b (m), x0 (n*n, may be NULL: zero start), xout (n*n).
+extern "C" int fair_mf_gd(fair_mf *h, const float *b, int32_t maxiter, const float *x0, double rho, double tol,
+                          double sigma, const fair_opts *o, float *xout, int32_t *iters) {
+    return guard([&] {
+        check_mf(h);
+        FAIR_CHECK(b && xout, FAIR_ERR_ARG, "null argument");
+        const int64_t n = (int64_t)h->n * h->n;
+        DBuf<float> db(h->m > 0 ? h->m : 1), dx(n);
+        db.upload(b, h->m);
+        if (x0) dx.upload(x0, n);
+        else FAIR_HIP(hipMemsetAsync(dx.p, 0, sizeof(float) * n, stream()));
+        const int st = fair_mf_gd_dev(h, db.p, maxiter, rho, tol, sigma, o, dx.p, iters);
+        if (st != FAIR_OK) throw Error{st, fair_last_error()};
+        dx.download(xout, n);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+    });
+}

@@ -14,7 +14,8 @@ module AIRMs
     use fair_capi
     use sparse_matrices
     implicit none
-    private :: initialise_options, set_upperbound, set_lowerbound, to_c
+    private :: initialise_options, set_upperbound, set_lowerbound
+    public :: to_c
 
     ! type(opts), AIRMs.f90:6-13
     type opts

@@ -188,6 +188,9 @@ int fair_mf_normest(fair_mf *P, const float *v0, double tol, int32_t maxit, doub
 /* gradient_descent, AIRMs.f90:860-995, on the operator; d_x: x0 in, result out. */
 int fair_mf_gd_dev(fair_mf *P, const float *d_b, int32_t maxiter, double rho, double tol, double sigma,
                    const fair_opts *opts, float *d_x, int32_t *iters);
+/* Host-array form (x0 may be NULL: zero start). */
+int fair_mf_gd(fair_mf *P, const float *b, int32_t maxiter, const float *x0, double rho, double tol,
+               double sigma, const fair_opts *opts, float *xout, int32_t *iters);
 
 #ifdef __cplusplus
 }
