@@ -94,11 +94,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
-    ap.add_argument("--mode", default="sart", choices=["sart", "precision", "batch", "matfree"],
+    ap.add_argument("--mode", default="sart", choices=["sart", "precision", "batch", "matfree", "kaczmarz"],
                     help="sart: the headline N=512 FP64 SART line (default); precision: FP32 vs FP64 "
                          "time to equal error at N=512; batch: 256 slices of 256^2 as one batched FP32 run; "
                          "matfree: matrix-free FP32 gradient descent at 2048^2 and at N=512 against the "
-                         "FP64 CSR method")
+                         "FP64 CSR method; kaczmarz: randomized (row-norm sampling) and symmetric "
+                         "Kaczmarz, 20 sweeps at N=256, against the oracle and the reference on the host")
     ap.add_argument("--mf-iters", type=int, default=100)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB fill and a 1 GiB sum (reference kernels for HBM counters)")
@@ -124,6 +125,8 @@ def main() -> int:
         return bench_batch(a, np, torch, _lib, airms, cf, sg, L, stream)
     if a.mode == "matfree":
         return bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream)
+    if a.mode == "kaczmarz":
+        return bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream)
 
     g = cf.C2
     dat = sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta())
@@ -446,6 +449,60 @@ def bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
                                         "matrix_free_fp32_s": round(t32, 4), "relerr": err,
                                         "within_1e-5": err <= 1e-5},
            "cpu_baseline_n512_gd": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
+    """Kaczmarz at N=256 (fan beam 360 x 363 rays, three-phase phantom, 1 % noise, seed 5), box
+    [0,1], 20 sweeps: randomized Kaczmarz with row-norm sampling (the reference's random_number
+    draws supplied from a seeded generator) and symmetric Kaczmarz (K=20: 10 forward + reverse
+    pairs).  Each GPU result is checked against the oracle run on the same draws; the reference's
+    own art (one host thread, as it is serial) is timed on the same problem."""
+    g = cf.C3
+    dat = sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta())
+    A = sg.fanlineartomo(dat)
+    IA, JA, vA = A.components()
+    xt = cf.threephases(g.n, seed=5)
+    b = cf.add_noise(A.matvecmul(xt), 0.01, 5)
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    nonempty = int(np.count_nonzero(np.diff(IA) > 0))
+    from oracle.oracle import Oracle, Reference, build_oracle
+    build_oracle()
+    orc = Oracle()
+    rows = []
+    for method, rowprob in (("randkaczmarz", True), ("symkaczmarz", False)):
+        o.rowprobability = rowprob
+        u = None
+        if method == "randkaczmarz":
+            u = np.random.default_rng(5).random(airms.art_uniform_count(method, A, [20], options=o))
+        airms.art(method, A, b, [1 if method == "randkaczmarz" else 2], options=o, uniforms=u)   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X = airms.art(method, A, b, [20], options=o, uniforms=u)
+        t_gpu = time.perf_counter() - t0
+        Xo, _ = orc.art(method, IA, JA, vA, A.n, b, [20], u=u, lb=0.0, ub=1.0, rowprob=rowprob)
+        err = float(np.linalg.norm(X[0] - Xo[0]) / np.linalg.norm(Xo[0]))
+        steps = 20 * nonempty
+        t_ref = None
+        if Reference.available() and not a.no_cpu:
+            ref = Reference()
+            ref.set(0, IA, JA, vA, A.n)
+            ref.rng_seed(5)
+            t0 = time.perf_counter()
+            ref.art(0, method, b, [20], lb=0.0, ub=1.0, rowprob=rowprob)
+            t_ref = time.perf_counter() - t0
+        log(f"{method}: GPU {t_gpu:.3f} s ({steps / t_gpu:.3g} rows/s), rel. error to oracle {err:.2e}"
+            + (f"; reference on one host thread {t_ref:.3f} s" if t_ref else ""))
+        rows.append({"method": method, "rowprobability": rowprob, "sweeps": 20, "row_steps": steps,
+                     "gpu_s": round(t_gpu, 4), "rows_per_s": round(steps / t_gpu, 1),
+                     "nnz_per_s": round(20 * A.nnz / t_gpu, 1), "relerr_vs_oracle": err,
+                     "within_1e-12": err <= 1e-12,
+                     "reference_host_s": None if t_ref is None else round(t_ref, 4)})
+    out = {"metric": METRIC, "mode": "kaczmarz", "unit": "rows/s", "dtype": "f64",
+           "config": {"workload": "N=256 fan beam 360 x 363 rays, Kaczmarz 20 sweeps, box [0,1]",
+                      "nnz": A.nnz, "nonempty_rows": nonempty},
+           "rows": rows}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -208,12 +208,36 @@ __global__ void __launch_bounds__(512) k_gd_decide(const double *part, GdState *
 }
 
 // ---- ART: persistent row-ordered sweep ------------------------------------------------------
-// One workgroup walks the row sequence strictly in order.  Per row: block-wide dot a_i.x
-// (AIRMs.f90:841), coeff = relax*(b_i - dot)/nrA_i (:842), then x(JA) += coeff*vA with
-// max(lb) / min(ub) on the touched entries (sparse_matrices.f90:808-820).  Column indices are
-// unique within a row, so each entry has one owner thread; the barrier after the update
-// orders it before the next row's dot.
-constexpr int SWEEP_THREADS = 512;
+// One wavefront walks the row sequence strictly in order (no barriers: its memory operations
+// are issued in program order, so a row's x stores precede the next row's x loads).  Per row:
+// dot a_i.x (AIRMs.f90:841) reduced with an xor butterfly, which leaves the identical sum in
+// every lane; coeff = relax*(b_i - dot)/nrA_i (:842); then x(JA) += coeff*vA with max(lb) /
+// min(ub) on the touched entries (sparse_matrices.f90:808-820).  Column indices are unique
+// within a row, so each entry has one owner lane.
+// Latency hiding: the entries of row s+1 (up to 64*ART_R of them) are loaded into registers while
+// row s is processed -- issued after row s's x gathers, so the in-order load counter lets those
+// gathers complete first -- and row descriptors run two steps ahead.  Longer rows load the
+// remainder inline.
+constexpr int SWEEP_THREADS = WAVE;
+constexpr int ART_R = 8;
+
+struct ArtRow {
+    int32_t r, beg, end;
+    double b, w;
+};
+
+// Loads are unconditional (clamped to valid addresses, results selected afterwards) so that the
+// compiler can wait for exactly the loads it needs instead of for all outstanding ones.
+__device__ __forceinline__ ArtRow art_row(const int32_t *rows, int64_t s, int64_t nsteps, const int32_t *IA,
+                                          const double *b, const double *nrA) {
+    ArtRow d;
+    d.r = rows[s < nsteps ? s : nsteps - 1] - 1;
+    d.beg = IA[d.r];
+    d.end = s < nsteps ? IA[d.r + 1] : d.beg;
+    d.b = b[d.r];
+    d.w = nrA[d.r];
+    return d;
+}
 
 __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(const int32_t *__restrict__ rows, int64_t nsteps,
                                                              const int32_t *__restrict__ IA,
@@ -223,22 +247,63 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(const int32_t *__re
                                                              const double *__restrict__ nrA,
                                                              int32_t ncols, double relax, int lwrb,
                                                              double lb, int upb, double ub, double *x) {
-    __shared__ double part[SWEEP_THREADS / WAVE];
-    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid >> 6;
-    for (int64_t s = 0; s < nsteps; ++s) {
-        const int r = rows[s] - 1;
-        const int32_t beg = IA[r], end = IA[r + 1];
-        double dot = 0.0;
-        for (int32_t k = beg + tid; k < end; k += SWEEP_THREADS)
-            if ((uint32_t)JA[k] < (uint32_t)ncols) dot += x[JA[k]] * vA[k];
-        dot = wave_sum(dot);
-        if (lane == 0) part[wv] = dot;
-        __syncthreads();
-        double inp = 0.0;
+    const int lane = threadIdx.x;
+    if (nsteps <= 0) return;
+    ArtRow cur = art_row(rows, 0, nsteps, IA, b, nrA);
+    ArtRow nxt = art_row(rows, 1, nsteps, IA, b, nrA);
+    int32_t jc[ART_R];
+    double vc[ART_R];
+    // JA / vA always have CSR_PAD spare entries, so index 0 is a valid address even for nnz = 0
 #pragma unroll
-        for (int w = 0; w < SWEEP_THREADS / WAVE; ++w) inp += part[w];
-        const double coeff = (relax * (b[r] - inp)) / nrA[r];
-        for (int32_t k = beg + tid; k < end; k += SWEEP_THREADS) {
+    for (int u = 0; u < ART_R; ++u) {
+        const int32_t k = cur.beg + lane + WAVE * u;
+        const int32_t kk = k < cur.end ? k : 0;
+        const int32_t jv = JA[kk];
+        const double vv = vA[kk];
+        jc[u] = k < cur.end ? jv : -1;
+        vc[u] = k < cur.end ? vv : 0.0;
+    }
+    for (int64_t s = 0; s < nsteps; ++s) {
+        // x gathers of this row (after the previous row's stores)
+        double xv[ART_R];
+#pragma unroll
+        for (int u = 0; u < ART_R; ++u) {
+            const bool ok = (uint32_t)jc[u] < (uint32_t)ncols;
+            const double xx = x[ok ? jc[u] : 0];
+            xv[u] = ok ? xx : 0.0;
+        }
+        // prefetch: entries of the next row, descriptor of the row after
+        int32_t jn[ART_R];
+        double vn[ART_R];
+#pragma unroll
+        for (int u = 0; u < ART_R; ++u) {
+            const int32_t k = nxt.beg + lane + WAVE * u;
+            const int32_t kk = k < nxt.end ? k : 0;
+            const int32_t jv = JA[kk];
+            const double vv = vA[kk];
+            jn[u] = k < nxt.end ? jv : -1;
+            vn[u] = k < nxt.end ? vv : 0.0;
+        }
+        const ArtRow nn = art_row(rows, s + 2, nsteps, IA, b, nrA);
+        double dot = 0.0;
+#pragma unroll
+        for (int u = 0; u < ART_R; ++u) dot += xv[u] * vc[u];
+        for (int32_t k = cur.beg + lane + WAVE * ART_R; k < cur.end; k += WAVE) {
+            const int j = JA[k];
+            if ((uint32_t)j < (uint32_t)ncols) dot += x[j] * vA[k];
+        }
+        dot = wave_sum(dot);
+        const double coeff = (relax * (cur.b - dot)) / cur.w;
+#pragma unroll
+        for (int u = 0; u < ART_R; ++u) {
+            if ((uint32_t)jc[u] < (uint32_t)ncols) {
+                double v = xv[u] + coeff * vc[u];
+                if (lwrb) v = fmax_ref(v, lb);
+                if (upb) v = fmin_ref(v, ub);
+                x[jc[u]] = v;
+            }
+        }
+        for (int32_t k = cur.beg + lane + WAVE * ART_R; k < cur.end; k += WAVE) {
             const int j = JA[k];
             if ((uint32_t)j >= (uint32_t)ncols) continue;
             double v = x[j] + coeff * vA[k];
@@ -246,7 +311,13 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(const int32_t *__re
             if (upb) v = fmin_ref(v, ub);
             x[j] = v;
         }
-        __syncthreads();
+        cur = nxt;
+        nxt = nn;
+#pragma unroll
+        for (int u = 0; u < ART_R; ++u) {
+            jc[u] = jn[u];
+            vc[u] = vn[u];
+        }
     }
 }
 
