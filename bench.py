@@ -94,12 +94,14 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
-    ap.add_argument("--mode", default="sart", choices=["sart", "precision", "batch", "matfree", "kaczmarz"],
+    ap.add_argument("--mode", default="sart", choices=["sart", "precision", "batch", "matfree", "kaczmarz", "uneven"],
                     help="sart: the headline N=512 FP64 SART line (default); precision: FP32 vs FP64 "
                          "time to equal error at N=512; batch: 256 slices of 256^2 as one batched FP32 run; "
                          "matfree: matrix-free FP32 gradient descent at 2048^2 and at N=512 against the "
                          "FP64 CSR method; kaczmarz: randomized (row-norm sampling) and symmetric "
-                         "Kaczmarz, 20 sweeps at N=256, against the oracle and the reference on the host")
+                         "Kaczmarz, 20 sweeps at N=256, against the oracle and the reference on the host; "
+                         "uneven: 64 SART problems from N=128 to N=1024 as one group against one run each")
+    ap.add_argument("--uneven-count", type=int, default=64)
     ap.add_argument("--mf-iters", type=int, default=100)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB fill and a 1 GiB sum (reference kernels for HBM counters)")
@@ -127,6 +129,8 @@ def main() -> int:
         return bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream)
     if a.mode == "kaczmarz":
         return bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream)
+    if a.mode == "uneven":
+        return bench_uneven(a, np, torch, _lib, airms, cf, sg, L, stream)
 
     g = cf.C2
     dat = sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta())
@@ -503,6 +507,73 @@ def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
            "config": {"workload": "N=256 fan beam 360 x 363 rays, Kaczmarz 20 sweeps, box [0,1]",
                       "nnz": A.nnz, "nonempty_rows": nonempty},
            "rows": rows}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_uneven(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
+    """The uneven batch: 64 seeded SART problems (N from 128 to 1024, N/2..2N angles, gallery
+    phantoms, 1 % noise, box [0,1]), each with its own device-built matrix, iterated 20 times as
+    one group (two persistent launches per iteration for all problems) against the same problems
+    run one after another; the group's iterates must equal the single runs bit for bit."""
+    dev = torch.device("cuda:0")
+    iters = 20
+    probs = cf.c5_problems(a.uneven_count, 5)
+    gens = [cf.shepplogan, cf.bubbles, cf.smooth, cf.threephases]
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    mats, db = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, g in enumerate(probs):
+        mats.append(sg.fanlineartomo(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta())))
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    for k, (g, A) in enumerate(zip(probs, mats)):
+        x = gens[k % 4](g.n, seed=5 + k) if gens[k % 4] is not cf.shepplogan else cf.shepplogan(g.n)
+        db.append(torch.from_numpy(cf.add_noise(A.matvecmul(x), 0.01, 5 + k)).to(dev))
+    nnz = sum(A.nnz for A in mats)
+    log(f"{len(mats)} problems, N {min(g.n for g in probs)}..{max(g.n for g in probs)}, total nnz {nnz:.4g}, "
+        f"device builds {t_build:.2f} s")
+    t0 = time.perf_counter()
+    G = airms.SirtGroup(mats, "sart")
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    wg = G.autotune()
+    xg = [torch.zeros(A.n, dtype=torch.float64, device=dev) for A in mats]
+    G.sirt_dev([b.data_ptr() for b in db], 1, [x.data_ptr() for x in xg], o)     # warm-up
+    for x in xg:
+        x.zero_()
+    torch.cuda.synchronize()
+    e0, e1 = _events(torch, stream)
+    e0.record(stream)
+    G.sirt_dev([b.data_ptr() for b in db], iters, [x.data_ptr() for x in xg], o)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_group = e0.elapsed_time(e1) / 1e3
+    log(f"group: {iters} iterations {t_group * 1e3:.1f} ms ({wg} workgroups per CU), set-up {t_setup:.2f} s")
+    xs = [torch.zeros(A.n, dtype=torch.float64, device=dev) for A in mats]
+    for A, b, x in zip(mats, db, xs):       # weights, layouts and workspaces, untimed
+        airms.sirt_dev("sart", A, b.data_ptr(), 1, x.data_ptr(), o)
+        x.zero_()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for A, b, x in zip(mats, db, xs):
+        airms.sirt_dev("sart", A, b.data_ptr(), iters, x.data_ptr(), o)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_seq = e0.elapsed_time(e1) / 1e3
+    same = all(bool(torch.equal(u, v)) for u, v in zip(xg, xs))
+    log(f"one problem after another: {t_seq * 1e3:.1f} ms; group iterates bit-identical: {same}")
+    bytes_it = sum(24 * A.nnz + 4 * (A.m + A.n + 2) + 32 * A.m + 32 * A.n for A in mats)
+    out = {"metric": METRIC, "mode": "uneven", "unit": "problem-iterations/s", "dtype": "f64",
+           "value": round(len(mats) * iters / t_group, 2),
+           "config": {"workload": f"{len(mats)} fan-beam SART problems, N 128..1024, N/2..2N angles, box [0,1], "
+                                  f"{iters} iterations, one group", "total_nnz": nnz,
+                      "sizes": [g.n for g in probs]},
+           "group_s": round(t_group, 5), "sequential_s": round(t_seq, 5), "speedup": round(t_seq / t_group, 3),
+           "group_gbs": round(bytes_it * iters / t_group / 1e9, 1), "workgroups_per_cu": wg,
+           "bit_identical_to_single_runs": same, "device_build_s": round(t_build, 3),
+           "group_setup_s": round(t_setup, 3)}
     print(json.dumps(out), flush=True)
     return 0
 

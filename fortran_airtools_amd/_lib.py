@@ -77,6 +77,10 @@ SIGNATURES = {
     "fair_art": [_P, _I32, _P, _I32, _P, _P, _P, _I32, C.POINTER(Opts), _P, _I64, _P],
     "fair_art_uniform_count": [_P, _I32, _I32, _P, _P, _I32, C.POINTER(Opts), C.POINTER(_I64)],
     "fair_art_steps_dev": [_P, _P, _I64, _P, _P, C.POINTER(Opts), _P],
+    "fair_group_create": [_I32, _P, _I32, C.POINTER(_P)],
+    "fair_group_free": [_P],
+    "fair_group_sirt_dev": [_P, _P, _I32, _P, C.POINTER(Opts)],
+    "fair_group_autotune": [_P, _P],
     "fair_mf_create": [C.POINTER(Geom), _P, C.POINTER(_P)],
     "fair_mf_free": [_P],
     "fair_mf_forward": [_P, _P, _P],

@@ -187,3 +187,44 @@ def art(art_method: str, A: CSRMatrix, b, Kin, x0=None, order=None, options: Opt
                          0 if ordv is None else ordv.shape[0], C.byref(o), _ptr(u),
                          0 if u is None else u.shape[0], _ptr(out)))
     return out
+
+
+class SirtGroup:
+    """Independent sirt problems (each its own CSRMatrix, sizes may differ) iterated together:
+    two launches per iteration for all of them, each iterate bit-identical to its own sirt run.
+    The matrices must outlive the group and stay unchanged."""
+
+    def __init__(self, mats, sirt_method: str = "sart"):
+        if sirt_method not in SIRT_METHODS:
+            raise FairError(FAIR_ERR_METHOD, "Not a recognised SIRT method.")
+        self.mats = list(mats)
+        arr = (C.c_void_p * len(self.mats))(*[A.handle for A in self.mats])
+        h = C.c_void_p()
+        check(lib().fair_group_create(len(self.mats), arr, SIRT_METHODS[sirt_method], C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            check(lib().fair_group_free(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self) -> int:
+        c = C.c_int32()
+        check(lib().fair_group_autotune(self.handle, C.byref(c)))
+        return c.value
+
+    def sirt_dev(self, d_bs, maxits: int, d_xs, options: Opts | None = None) -> None:
+        """maxits iterations; d_bs / d_xs: device addresses (ints) per problem, x updated in place."""
+        if len(d_bs) != len(self.mats) or len(d_xs) != len(self.mats):
+            raise FairError(FAIR_ERR_ARG, "one b and one x per problem")
+        pb = (C.c_void_p * len(d_bs))(*d_bs)
+        px = (C.c_void_p * len(d_xs))(*d_xs)
+        o = _opts(options)
+        check(lib().fair_group_sirt_dev(self.handle, pb, int(maxits), px, C.byref(o)))
+

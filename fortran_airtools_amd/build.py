@@ -21,12 +21,14 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(ROOT, "include")]
-# The FP64 reference-path files round every product and sum separately, as the reference's
-# x86-64 build (no FMA) does.  The matrix-free projector is built the same way so that its
+# The FP64 reference-path files (and the grouped SIRT, whose iterates equal the single-problem
+# ones bit for bit) round every product and sum separately, as the reference's x86-64 build (no
+# FMA) does.  The matrix-free projector is built the same way so that its
 # forward and adjoint kernels evaluate the ray geometry with identical roundings (fused
 # multiply-adds are written out where wanted).
 PER_FILE = {"raytrace.hip": ["-ffp-contract=off"], "sparse.hip": ["-ffp-contract=off"],
-            "methods.hip": ["-ffp-contract=off"], "matfree.hip": ["-ffp-contract=off"]}
+            "methods.hip": ["-ffp-contract=off"], "group.hip": ["-ffp-contract=off"],
+            "matfree.hip": ["-ffp-contract=off"]}
 
 
 def _compile(src: str) -> str:

@@ -59,6 +59,23 @@ C3 = Geometry(n=256, p=363, nth=360, deg_step=1.0)
 # C4: matrix-free (N=2048, 2048 angles over 360 deg, 2900 rays).
 C4 = Geometry(n=2048, p=2900, nth=2048, deg_step=360.0 / 2048)
 
+# C5: uneven batch -- N drawn from these sizes, angles uniform in N/2..2N over 360 degrees, and
+# p the odd number nearest above N/2 (the odd-p fan of the reference, about N/2 detector rays).
+C5_SIZES = (128, 192, 256, 384, 512, 768, 1024)
+
+
+def c5_problems(count: int = 64, seed: int = 5) -> list:
+    """The uneven problem set: `count` geometries from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(C5_SIZES))
+        nth = int(rng.integers(n // 2, 2 * n + 1))
+        h = n // 2
+        p = h + 1 if h % 2 == 0 else h
+        out.append(Geometry(n=n, p=p, nth=nth, deg_step=360.0 / nth))
+    return out
+
 
 def linspace(a: float, b: float, n: int) -> np.ndarray:
     """system_generation.f90:17-35: x(i) = c*i + d with c=(b-a)/(n-1), d=a-c."""

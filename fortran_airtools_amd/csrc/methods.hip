@@ -336,7 +336,7 @@ void check_handle(const fair_csr *A) { FAIR_CHECK(A != nullptr, FAIR_ERR_ARG, "n
 // SIRT weights (AIRMs.f90:1070-1083), computed once per matrix and cached in the handle.
 // Cimmino: M = m*||a_i||^2, D = 1.  SART: M = row sums, then the reference's double transpose
 // (which leaves A's rows column-sorted), D = column sums.
-void sirt_setup(fair_csr *h, int method) {
+void sirt_weights_impl(fair_csr *h, int method) {
     FAIR_CHECK(method == FAIR_CIMMINO || method == FAIR_SART, FAIR_ERR_METHOD, "Not a recognised SIRT method.");
     Csr &A = h->A;
     if (A.n_rows_nz < 0) build_row_list(A);
@@ -357,6 +357,11 @@ void sirt_setup(fair_csr *h, int method) {
         h->w_ok[method] = true;
     }
     ensure_T(h);
+}
+
+void sirt_setup(fair_csr *h, int method) {
+    sirt_weights_impl(h, method);
+    const int32_t m = h->A.m, n = h->A.n;
     ensure_blocked(h);
     if (h->ws_r.n < (size_t)(m > 0 ? m : 1)) h->ws_r.alloc(m > 0 ? m : 1);
     if (h->ws_x.n < (size_t)(n > 0 ? n : 1)) h->ws_x.alloc(n > 0 ? n : 1);
@@ -395,6 +400,7 @@ void sirt_run(fair_csr *h, int method, const double *d_b, int maxits, double *d_
 }  // namespace
 
 void sirt_setup_public(fair_csr *h, int method) { sirt_setup(h, method); }
+void sirt_weights(fair_csr *h, int method) { sirt_weights_impl(h, method); }
 
 }  // namespace fair
 

@@ -152,17 +152,16 @@ __global__ void __launch_bounds__(256) k_spmv_wave(const int4 *__restrict__ rows
 // so the matrix stream and the footprint loads are in flight together.
 constexpr int SPMV_BLOCK_THREADS = 512;
 
-template <class V, class Src, class Epi>
-__global__ void __launch_bounds__(SPMV_BLOCK_THREADS) k_spmv_blocked(
-    const int4 *__restrict__ binfo, const int4 *__restrict__ rows, const int2 *__restrict__ segs,
-    const uint16_t *__restrict__ loc, const int32_t *__restrict__ JA, const V *__restrict__ vA,
-    int32_t ncols, int32_t zero_slot, Src src, Epi epi) {
+// One block of rows (blocked layout) by a workgroup of SPMV_BLOCK_THREADS; lx: LDS of the
+// block's footprint (zero_slot + 1 entries).  Shared by the single-matrix kernel and the grouped
+// one (group.hip), so both compute every row with the same operations in the same order.
+template <class V, class Epi>
+__device__ __forceinline__ void spmv_block(const int4 bi, const int4 *__restrict__ rows,
+                                           const int2 *__restrict__ segs, const uint16_t *__restrict__ loc,
+                                           const int32_t *__restrict__ JA, const V *__restrict__ vA,
+                                           int32_t ncols, int32_t zero_slot, const V *__restrict__ x, Epi epi,
+                                           V *lx) {
     constexpr int NW = SPMV_BLOCK_THREADS / WAVE;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    V *lx = (V *)smem_raw;
-    const V *__restrict__ x = src();
-    if (!x) return;
-    const int4 bi = binfo[blockIdx.x];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & (WAVE - 1);
     if (bi.z >= 0) {
         int q = bi.x + wv;
@@ -206,6 +205,17 @@ __global__ void __launch_bounds__(SPMV_BLOCK_THREADS) k_spmv_blocked(
             if (lane == 0) epi(ri.x, acc);
         }
     }
+}
+
+template <class V, class Src, class Epi>
+__global__ void __launch_bounds__(SPMV_BLOCK_THREADS) k_spmv_blocked(
+    const int4 *__restrict__ binfo, const int4 *__restrict__ rows, const int2 *__restrict__ segs,
+    const uint16_t *__restrict__ loc, const int32_t *__restrict__ JA, const V *__restrict__ vA,
+    int32_t ncols, int32_t zero_slot, Src src, Epi epi) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const V *__restrict__ x = src();
+    if (!x) return;
+    spmv_block<V, Epi>(binfo[blockIdx.x], rows, segs, loc, JA, vA, ncols, zero_slot, x, epi, (V *)smem_raw);
 }
 
 template <class V, class Src, class Epi>

@@ -165,6 +165,21 @@ int fair_art_uniform_count(fair_csr *A, int32_t method, int32_t nk, const int32_
 int fair_art_steps_dev(fair_csr *A, const int32_t *d_rows, int64_t nsteps, const double *d_b,
                        const double *d_nrA, const fair_opts *opts, double *d_x);
 
+/* ---- groups of independent SIRT problems (uneven batch) ------------------------------------
+ * No counterpart in the reference (its to-do "blocking support", README.md:6).  Problems with
+ * their own matrices (different sizes) are iterated together: each sirt iteration
+ * (AIRMs.f90:1089-1110) of every problem is two persistent launches over all problems' rows, and
+ * every problem's iterate is bit-identical to its own fair_sirt_dev run.  The matrices must
+ * outlive the group and stay unchanged. */
+typedef struct fair_group fair_group;
+int fair_group_create(int32_t nprob, fair_csr *const *As, int32_t method, fair_group **G);
+int fair_group_free(fair_group *G);
+/* maxits iterations of every problem: d_b[p] (m_p), d_x[p] (n_p; start in, result out) in HBM. */
+int fair_group_sirt_dev(fair_group *G, const double *const *d_b, int32_t maxits, double *const *d_x,
+                        const fair_opts *opts);
+/* Picks the persistent grid size (workgroups per CU) by timing; wg_per_cu may be NULL. */
+int fair_group_autotune(fair_group *G, int32_t *wg_per_cu);
+
 /* ---- matrix-free fan-beam projector (FP32) ------------------------------------------------
  * The reference's to-do "make matrix-free implementations" (README.md:7): the operator of
  * fanlineartomo (system_generation.f90:79-307) with the same angle and ray tables, its
