@@ -72,6 +72,7 @@ SIGNATURES = {
     "fair_batch_relerr_dev": [_P, _P, _I32, _I32, _P],
     "fair_sirt_batch": [_P, _I32, _P, _I32, _I32, _P, C.POINTER(Opts), _I32, _P],
     "fair_autotune_batch": [_P, _I32, _P],
+    "fair_autotune_spmv": [_P, _P],
     "fair_normest": [_P, _P, _D, _I32, _P],
     "fair_gradient_descent": [_P, _P, _I32, _P, _D, _D, _D, C.POINTER(Opts), _P, _P],
     "fair_art": [_P, _I32, _P, _I32, _P, _P, _P, _I32, C.POINTER(Opts), _P, _I64, _P],

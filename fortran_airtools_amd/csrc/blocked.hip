@@ -17,6 +17,7 @@
 #include <numeric>
 
 #include "fair_internal.hpp"
+#include "spmv.hpp"
 
 namespace fair {
 namespace {
@@ -127,7 +128,10 @@ void build_blocked(Csr &A, const std::vector<int64_t> &key_of_row, int32_t rb, i
     B = Blocked();
     if (A.n_rows_nz < 0) build_row_list(A);
     const int32_t nr = A.n_rows_nz;
-    if (nr <= 0 || NB > 8192 || lds_elems > 65535 || rb < 1) return;
+    // 16-bit LDS slots; the FP64 footprint must fit the workgroup's LDS
+    if (nr <= 0 || NB > 8192 || lds_elems > 65535 || rb < 1 ||
+        (int64_t)lds_elems * (int64_t)sizeof(double) > (int64_t)lds_per_block())
+        return;
     std::vector<int4> desc(nr);
     FAIR_HIP(hipMemcpyAsync(desc.data(), A.rows_nz.p, sizeof(int4) * nr, hipMemcpyDeviceToHost, stream()));
     FAIR_HIP(hipStreamSynchronize(stream()));
@@ -221,10 +225,11 @@ void build_blocked_tiles(Csr &T, int32_t img_n, int32_t p, int32_t nth, int32_t 
 }
 
 // Build the staged layouts of a ray-traced matrix and of its transpose (once).  The layouts
-// can be switched off with FAIR_SPMV_STAGED=0; FAIR_RB / FAIR_TILE / FAIR_LDS override the
-// block shape.
+// can be switched off with FAIR_SPMV_STAGED=0; FAIR_DA / FAIR_DJ / FAIR_LDS_A and FAIR_TX /
+// FAIR_TY / FAIR_LDS_T override the default block shapes (fair_autotune_spmv picks them by
+// timing).
 void ensure_blocked(fair_csr *h) {
-    if (h->g_n <= 0 || h->transposed || !env_int("FAIR_SPMV_STAGED", 1)) return;
+    if (h->g_n <= 0 || h->transposed || h->blk_tuned || !env_int("FAIR_SPMV_STAGED", 1)) return;
     const bool built = !h->A.blk.ok || !h->have_T || !h->T.blk.ok;
     if (!h->A.blk.ok)
         build_blocked_rays(h->A, h->g_n, h->g_p, env_int("FAIR_DA", 4), env_int("FAIR_DJ", 4),
@@ -240,3 +245,124 @@ void ensure_blocked(fair_csr *h) {
 }
 
 }  // namespace fair
+
+namespace fair {
+namespace {
+
+// Average time of one A*x launch of the staged kernel (3 launches after one warm-up).
+float time_blocked(const Csr &A, const double *x, double *y, hipEvent_t e0, hipEvent_t e1) {
+    auto run = [&] {
+        spmv_launch_blocked<double, SrcPtr<double>, EpiStore<double>>(A.blk, A.JA.p, A.vA.p, A.n, SrcPtr<double>{x},
+                                                                     EpiStore<double>{y}, stream());
+    };
+    run();
+    FAIR_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < 3; ++r) run();
+    FAIR_HIP(hipEventRecord(e1, stream()));
+    FAIR_HIP(hipEventSynchronize(e1));
+    launch_check("spmv autotune");
+    float ms = 0.f;
+    FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return ms / 3.f;
+}
+
+float time_direct(const Csr &A, const double *x, double *y, hipEvent_t e0, hipEvent_t e1) {
+    auto run = [&] {
+        spmv_launch_raw<double, SrcPtr<double>, EpiStore<double>>((const int4 *)A.rows_nz.p, A.n_rows_nz, A.JA.p,
+                                                                  A.vA.p, A.n, SrcPtr<double>{x}, EpiStore<double>{y},
+                                                                  stream());
+    };
+    run();
+    FAIR_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < 3; ++r) run();
+    FAIR_HIP(hipEventRecord(e1, stream()));
+    FAIR_HIP(hipEventSynchronize(e1));
+    launch_check("spmv autotune");
+    float ms = 0.f;
+    FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return ms / 3.f;
+}
+
+struct Shape {
+    int32_t a, b, lds;
+};
+
+}  // namespace
+}  // namespace fair
+
+using namespace fair;
+
+// Block shapes of the staged SpMV picked by timing on this matrix: sinogram tiles of DA angles x
+// DJ detector positions for A, image tiles of TX x TY pixels for A^T, each with an LDS footprint
+// capacity.  choice (may be NULL) receives {DA, DJ, LDS_A, TX, TY, LDS_T}.  No counterpart in the
+// reference (its products are OpenMP loops over rows, sparse_matrices.f90:349-393).
+extern "C" int fair_autotune_spmv(fair_csr *h, int32_t *choice) {
+    return guard([&] {
+        FAIR_CHECK(h != nullptr, FAIR_ERR_ARG, "null matrix handle");
+        FAIR_CHECK(h->g_n > 0 && !h->transposed, FAIR_ERR_ARG,
+                   "the staged layout needs a matrix built by fanlineartomo or paralleltomo");
+        h->blk_tuned = false;
+        ensure_blocked(h);
+        FAIR_CHECK(h->A.blk.ok && h->T.blk.ok, FAIR_ERR_ARG, "the staged layout is switched off");
+        if (h->graphs_used) drop_graphs(h);   // captured launches refer to the old layouts
+        Csr &A = h->A, &T = h->T;
+        const int32_t big = std::max(A.m, A.n);
+        DBuf<double> x(big > 0 ? big : 1), y(big > 0 ? big : 1);
+        fill(x.p, big, 1.0);
+        hipEvent_t e0, e1;
+        FAIR_HIP(hipEventCreate(&e0));
+        FAIR_HIP(hipEventCreate(&e1));
+        // {0, 0, 0}: no staging (row-per-wavefront kernel with global gathers)
+        const Shape ray_shapes[] = {{4, 4, 6144}, {2, 8, 6144}, {4, 8, 6144}, {8, 4, 6144}, {8, 2, 6144},
+                                    {16, 1, 6144}, {16, 2, 6144}, {32, 1, 6144}, {8, 8, 12288}, {0, 0, 0}};
+        const Shape tile_shapes[] = {{8, 8, 6144}, {8, 16, 6144}, {16, 8, 6144}, {4, 8, 4096},
+                                     {16, 16, 12288}, {8, 16, 12288}, {0, 0, 0}};
+        auto time_any = [&](const Csr &M) {
+            if (M.blk.ok) return time_blocked(M, x.p, y.p, e0, e1);
+            return time_direct(M, x.p, y.p, e0, e1);
+        };
+        Shape bestA = ray_shapes[0], bestT = tile_shapes[0];
+        float tA = 1e30f, tT = 1e30f;
+        for (const Shape &c : ray_shapes) {
+            if (c.a > 0) {
+                build_blocked_rays(A, h->g_n, h->g_p, c.a, c.b, c.lds);
+                if (!A.blk.ok) continue;
+            } else {
+                A.blk = Blocked();
+            }
+            const float t = time_any(A);
+            if (t < tA) {
+                tA = t;
+                bestA = c;
+            }
+        }
+        for (const Shape &c : tile_shapes) {
+            if (c.a > 0) {
+                build_blocked_tiles(T, h->g_n, h->g_p, h->g_nth, c.a, c.b, c.lds);
+                if (!T.blk.ok) continue;
+            } else {
+                T.blk = Blocked();
+            }
+            const float t = time_any(T);
+            if (t < tT) {
+                tT = t;
+                bestT = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (bestA.a > 0) build_blocked_rays(A, h->g_n, h->g_p, bestA.a, bestA.b, bestA.lds);
+        else A.blk = Blocked();
+        if (bestT.a > 0) build_blocked_tiles(T, h->g_n, h->g_p, h->g_nth, bestT.a, bestT.b, bestT.lds);
+        else T.blk = Blocked();
+        FAIR_CHECK((bestA.a == 0 || A.blk.ok) && (bestT.a == 0 || T.blk.ok), FAIR_ERR_INTERNAL,
+                   "staged layout rebuild failed");
+        h->blk_tuned = true;
+        if (choice) {
+            const int32_t c[6] = {bestA.a, bestA.b, bestA.lds, bestT.a, bestT.b, bestT.lds};
+            for (int q = 0; q < 6; ++q) choice[q] = c[q];
+        }
+        FAIR_HIP(hipStreamSynchronize(stream()));
+    });
+}
+

@@ -136,6 +136,7 @@ struct fair_csr {
     // Geometry of a ray-traced matrix (image side, rays per angle, angles); 0 when unknown.
     int32_t g_n = 0, g_p = 0, g_nth = 0;
     bool transposed = false;   // fair_transpose swapped the roles of A and T
+    bool blk_tuned = false;    // staged layouts chosen by fair_autotune_spmv (possibly none)
     // SIRT weights cached across calls (index: FAIR_CIMMINO / FAIR_SART) and loop workspace;
     // dropped whenever the matrix changes.
     fair::DBuf<double> M[2], D;

@@ -5,14 +5,16 @@
 // every problem, work items over its A and its A^T: the blocks of their LDS-staged layouts
 // (blocked.hip), or runs of 8 rows where a matrix has no such layout, and runs of A^T's empty
 // rows (never-hit columns still get their box projection).  One SIRT iteration of every problem
-// is two launches of a persistent grid of 512-thread workgroups that strides over the items:
+// is two passes, each one launch of a persistent grid of 512-thread workgroups that strides over
+// the items (one launch per LDS size class when the layouts differ in footprint capacity):
 //   pass 0  r_p[i] = (b_p[i] - a_i.x_p) / M_p[i]                   (AIRMs.f90:1090 inner part)
 //   pass 1  x_p[j] = box(x_p[j] + relax * (A_p^T r_p)_j / D_p[j])   (:1090 outer part, :1093-1100)
 // A block is processed by the same device function as in the single-matrix kernel (spmv.hpp,
 // spmv_block), so every problem's iterate is bit-identical to its own sirt run.  Small problems
-// no longer leave the device idle, and an iteration costs two launches however many problems
+// no longer leave the device idle, and an iteration costs a few launches however many problems
 // there are.  The persistent grid size (workgroups per CU) is autotuned.
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -113,15 +115,20 @@ __global__ void __launch_bounds__(SPMV_BLOCK_THREADS) k_group_pass(const int4 *_
 }  // namespace
 }  // namespace fair
 
+// Items of one pass that need the same LDS footprint capacity (one launch each).
+struct fair_group_class {
+    size_t lds_bytes = 8;
+    fair::DBuf<int32_t> items;   // int4 each
+    int64_t n = 0;
+};
+
 struct fair_group {
     int32_t nprob = 0, method = 0;
     std::vector<fair_csr *> mats;
     std::vector<fair::GroupProb> host;
     fair::DBuf<fair::GroupProb> probs;
-    fair::DBuf<int32_t> itemsA, itemsT;   // int4 each
-    int64_t nA = 0, nT = 0;
+    std::vector<fair_group_class> passA, passT;
     std::vector<fair::DBuf<double>> r;
-    size_t lds_bytes = 8;
     int wg_per_cu = 2;
     int ncu = 1;
 };
@@ -142,12 +149,12 @@ void group_iterate(fair_group *g, int maxits, const fair_opts *o) {
     }
     const unsigned grid = (unsigned)std::max(1, g->ncu * g->wg_per_cu);
     for (int it = 0; it < maxits; ++it) {
-        if (g->nA > 0)
-            hipLaunchKernelGGL(k_group_pass<0>, dim3(grid), dim3(SPMV_BLOCK_THREADS), g->lds_bytes, stream(),
-                               (const int4 *)g->itemsA.p, g->nA, g->probs.p, relax, box);
-        if (g->nT > 0)
-            hipLaunchKernelGGL(k_group_pass<1>, dim3(grid), dim3(SPMV_BLOCK_THREADS), g->lds_bytes, stream(),
-                               (const int4 *)g->itemsT.p, g->nT, g->probs.p, relax, box);
+        for (const fair_group_class &c : g->passA)
+            hipLaunchKernelGGL(k_group_pass<0>, dim3(grid), dim3(SPMV_BLOCK_THREADS), c.lds_bytes, stream(),
+                               (const int4 *)c.items.p, c.n, g->probs.p, relax, box);
+        for (const fair_group_class &c : g->passT)
+            hipLaunchKernelGGL(k_group_pass<1>, dim3(grid), dim3(SPMV_BLOCK_THREADS), c.lds_bytes, stream(),
+                               (const int4 *)c.items.p, c.n, g->probs.p, relax, box);
     }
     launch_check("group sirt");
 }
@@ -178,20 +185,38 @@ GMat gmat(const Csr &A) {
     return m;
 }
 
-// Work items of one matrix of problem p; returns the LDS bytes its blocks need.
-size_t add_items(std::vector<int4> &items, int p, const Csr &A, bool with_empty) {
-    size_t lds = 0;
+// Work items of one matrix of problem p, filed by the LDS bytes they need (rows and empty rows
+// need none).
+void add_items(std::map<size_t, std::vector<int4>> &cls, int p, const Csr &A, bool with_empty) {
     if (A.blk.ok) {
+        std::vector<int4> &items = cls[sizeof(double) * (size_t)A.blk.lds_elems];
         for (int32_t b = 0; b < A.blk.nblocks; ++b) items.push_back(make_int4(p, ITEM_BLOCK, b, 0));
-        lds = sizeof(double) * (size_t)A.blk.lds_elems;
     } else {
+        std::vector<int4> &items = cls[0];
         for (int32_t r = 0; r < A.n_rows_nz; r += ROWS_PER_ITEM)
             items.push_back(make_int4(p, ITEM_ROWS, r, std::min(r + ROWS_PER_ITEM, A.n_rows_nz)));
     }
-    if (with_empty)
+    if (with_empty) {
+        std::vector<int4> &items = cls[0];
         for (int32_t r = 0; r < A.n_rows_empty; r += EMPTY_PER_ITEM)
             items.push_back(make_int4(p, ITEM_EMPTY, r, std::min(r + EMPTY_PER_ITEM, A.n_rows_empty)));
-    return lds;
+    }
+}
+
+void upload_classes(const std::map<size_t, std::vector<int4>> &cls, std::vector<fair_group_class> &out) {
+    out.clear();
+    out.resize(cls.size());
+    size_t q = 0;
+    for (const auto &kv : cls) {
+        fair_group_class &c = out[q++];
+        FAIR_CHECK(kv.first <= (size_t)lds_per_block(), FAIR_ERR_ARG, "block footprint exceeds the LDS");
+        c.lds_bytes = std::max<size_t>(kv.first, 8);
+        c.n = (int64_t)kv.second.size();
+        c.items.alloc(4 * std::max<size_t>(kv.second.size(), 1));
+        if (!kv.second.empty())
+            FAIR_HIP(hipMemcpyAsync(c.items.p, kv.second.data(), sizeof(int4) * kv.second.size(),
+                                    hipMemcpyHostToDevice, stream()));
+    }
 }
 
 }  // namespace
@@ -209,7 +234,7 @@ extern "C" int fair_group_create(int32_t nprob, fair_csr *const *As, int32_t met
         g->mats.assign(As, As + nprob);
         g->host.resize(nprob);
         g->r.resize(nprob);
-        std::vector<int4> iA, iT;
+        std::map<size_t, std::vector<int4>> iA, iT;
         for (int p = 0; p < nprob; ++p) {
             fair_csr *h = As[p];
             FAIR_CHECK(h != nullptr, FAIR_ERR_ARG, "null matrix in group");
@@ -221,18 +246,11 @@ extern "C" int fair_group_create(int32_t nprob, fair_csr *const *As, int32_t met
             GroupProb &q = g->host[p];
             q = GroupProb{gmat(A), gmat(T), h->M[method].p, method == FAIR_SART ? h->D.p : nullptr, g->r[p].p,
                           nullptr, nullptr};
-            g->lds_bytes = std::max(g->lds_bytes, add_items(iA, p, A, false));
-            g->lds_bytes = std::max(g->lds_bytes, add_items(iT, p, T, true));
+            add_items(iA, p, A, false);
+            add_items(iT, p, T, true);
         }
-        FAIR_CHECK(g->lds_bytes <= (size_t)lds_per_block(), FAIR_ERR_ARG, "block footprint exceeds the LDS");
-        g->nA = (int64_t)iA.size();
-        g->nT = (int64_t)iT.size();
-        g->itemsA.alloc(4 * std::max<size_t>(iA.size(), 1));
-        g->itemsT.alloc(4 * std::max<size_t>(iT.size(), 1));
-        if (!iA.empty())
-            FAIR_HIP(hipMemcpyAsync(g->itemsA.p, iA.data(), sizeof(int4) * iA.size(), hipMemcpyHostToDevice, stream()));
-        if (!iT.empty())
-            FAIR_HIP(hipMemcpyAsync(g->itemsT.p, iT.data(), sizeof(int4) * iT.size(), hipMemcpyHostToDevice, stream()));
+        upload_classes(iA, g->passA);
+        upload_classes(iT, g->passT);
         g->probs.alloc(nprob);
         int dev = 0;
         FAIR_HIP(hipGetDevice(&dev));

@@ -359,6 +359,7 @@ void invalidate_T(fair_csr *h) {
     if (h->have_T) {
         h->T = Csr();
         h->have_T = false;
+        h->blk_tuned = false;
     }
 }
 

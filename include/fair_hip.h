@@ -97,6 +97,11 @@ int fair_multiply_by_scalar(fair_csr *A, double c);
 int fair_add_scalar_mult_row(fair_csr *A, int32_t k, double c, double *w, int32_t nw,
                              const double *lwrb, const double *upb);
 
+/* Block shapes of the LDS-staged SpMV for A and A^T of a ray-traced matrix, picked by timing
+ * (no counterpart in the reference, whose products are OpenMP loops over rows,
+ * sparse_matrices.f90:349-393).  choice (may be NULL) receives {DA, DJ, LDS_A, TX, TY, LDS_T}. */
+int fair_autotune_spmv(fair_csr *A, int32_t *choice);
+
 /* Device-pointer forms of matvecmul / transvecmul (x, y resident in HBM). */
 int fair_matvecmul_dev(fair_csr *A, const double *d_v, double *d_w);
 int fair_transvecmul_dev(fair_csr *A, const double *d_v, double *d_w);

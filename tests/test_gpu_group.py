@@ -65,3 +65,22 @@ def test_group_continues_from_start_vectors():
     torch.cuda.synchronize()
     for a, b in zip(x1, x2):
         assert a.cpu().numpy().tobytes() == b.cpu().numpy().tobytes()
+
+
+def test_spmv_autotune_keeps_results():
+    """Re-tiling the staged layouts changes no result: rows keep their accumulation order."""
+    import ctypes as C
+    from fortran_airtools_amd import _lib, airms, configs as cf, system_generation as sg
+    g = cf.C1
+    A = sg.fanlineartomo(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta()))
+    x = np.random.default_rng(1).random(A.n)
+    y = np.random.default_rng(2).random(A.m)
+    b = A.matvecmul(cf.shepplogan(g.n))
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    before = (A.matvecmul(x), A.transvecmul(y), airms.sirt("sart", A, b, [5], options=o))
+    choice = (C.c_int32 * 6)()
+    _lib.check(_lib.lib().fair_autotune_spmv(A.handle, choice))
+    assert list(choice)[0] >= 0
+    after = (A.matvecmul(x), A.transvecmul(y), airms.sirt("sart", A, b, [5], options=o))
+    for u, v in zip(before, after):
+        assert u.tobytes() == v.tobytes()
