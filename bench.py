@@ -102,6 +102,7 @@ def main() -> int:
                          "Kaczmarz, 20 sweeps at N=256, against the oracle and the reference on the host; "
                          "uneven: 64 SART problems from N=128 to N=1024 as one group against one run each")
     ap.add_argument("--uneven-count", type=int, default=64)
+    ap.add_argument("--batch-iters", type=int, default=300)
     ap.add_argument("--mf-iters", type=int, default=100)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB fill and a 1 GiB sum (reference kernels for HBM counters)")
@@ -286,7 +287,7 @@ def bench_batch(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     SIRT (SART weights) run of 300 iterations captured in a HIP graph, against the plain FP64
     method run slice by slice; per-slice relative error of the batched result."""
     g = cf.C3
-    k, iters = 256, 300
+    k, iters = 256, a.batch_iters
     A = sg.fanlineartomo(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta()))
     gens = [cf.threephases, cf.bubbles, cf.smooth]
     dev = torch.device("cuda:0")
@@ -340,7 +341,7 @@ def bench_batch(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     flops = 2 * 2 * nnz * k * iters
     out = {"metric": METRIC, "mode": "batch", "unit": "slice-iterations/s", "dtype": "f32 (fp64 weights)",
            "value": round(k * iters / t_batch, 2),
-           "config": {"workload": "256 slices of 256x256, fan beam 360 x 363 rays, SART as SpMM, 300 iterations, "
+           "config": {"workload": f"256 slices of 256x256, fan beam 360 x 363 rays, SART as SpMM, {iters} iterations, "
                                   "HIP graph", "nnz": nnz, "slices": k, "iterations": iters},
            "batched_fp32_s": round(t_batch, 4), "fp64_slice_by_slice_s": round(t64, 4),
            "speedup_vs_fp64": round(t64 / t_batch, 3),

@@ -200,13 +200,6 @@ __global__ void __launch_bounds__(256) k_fill(RayArgs g, int64_t m, const int32_
     if (c != IA[ray + 1] - off) atomicOr(err, 2);
 }
 
-// Irregular rays: the reference algorithm verbatim (t arrays, Hoare quicksort with middle
-// pivot, permutation, filter, de-duplication), one thread per ray, arrays in global scratch.
-struct ExactScratch {
-    double *t, *xx, *yy;   // (2n+2) each, one-based addressing via pointer - 1
-    int32_t *ind, *stk;    // (2n+2) and 2*(2n+2)
-};
-
 // The reference's algorithm for one ray; base: 36 * (2n+2) bytes of scratch.  Returns the
 // number of entries (columns of any value) the ray produces.
 template <int BEAM, bool FILL, class E>
@@ -262,6 +255,93 @@ __device__ int exact_ray(const RayArgs &g, int64_t ray, char *base, E &e, bool *
     for (int k = 1; k <= nn; ++k) s.template push<FILL, E>(xx[ind[k]], yy[ind[k]], r.excl, e);
     if (s.bad_col) *bad_col = true;
     return s.cnt;
+}
+
+// The same algorithm with its sort in LDS, one 64-lane workgroup per ray: the lanes compute the
+// 2n+2 crossing parameters, lane 0 sorts (t, ind) with the reference's quicksort and runs the
+// point rule, recomputing each point from its index as the reference's arrays hold it
+// (system_generation.f90:162-170).  LDS: t (8 B), ind (4 B) and the range stack (8 B) per
+// crossing.  Shorter latency than global scratch by an order of magnitude.
+constexpr int EXACT_LDS_PER_CROSSING = 20;
+
+template <int BEAM, bool FILL, class E>
+__device__ int exact_ray_lds(const RayArgs &g, int64_t ray, unsigned char *lds, E &e, bool *bad_col) {
+    const int n = g.n, nn = 2 * n + 2;
+    double *t = (double *)lds - 1;                              // t[1..nn]
+    int32_t *stk = (int32_t *)(lds + 8 * (size_t)nn);          // 2*nn entries
+    int32_t *ind = (int32_t *)(lds + 16 * (size_t)nn) - 1;     // ind[1..nn]
+    const RayGeom r = ray_geom<BEAM>(g, ray);
+    const double half = .5 * (double)n;
+    for (int k = 1 + (int)threadIdx.x; k <= n + 1; k += (int)blockDim.x) {
+        const double xg = gridc(k, half);
+        t[k] = (xg - r.x0th) / r.a;
+        t[n + 1 + k] = (xg - r.y0th) / r.b;
+        ind[k] = k;
+        ind[n + 1 + k] = n + 1 + k;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return 0;
+    int sp = 0;
+    stk[sp++] = 1;
+    stk[sp++] = nn;
+    while (sp > 0) {
+        int l = stk[--sp], f = stk[--sp];
+        double x = t[(f + l) / 2];
+        int i = f, j = l;
+        for (;;) {
+            while (i < l && t[i] < x) ++i;
+            while (j > f && x < t[j]) --j;
+            if (i >= j) break;
+            double tt = t[i]; t[i] = t[j]; t[j] = tt;
+            int32_t mm = ind[i]; ind[i] = ind[j]; ind[j] = mm;
+            ++i;
+            --j;
+        }
+        if (j + 1 < l) { stk[sp++] = j + 1; stk[sp++] = l; }
+        if (f < i - 1) { stk[sp++] = f; stk[sp++] = i - 1; }
+    }
+    PointSink s;
+    s.lo = -half;
+    s.hi = half;
+    s.nd = (double)n;
+    s.n = n;
+    for (int k = 1; k <= nn; ++k) {
+        const int q = ind[k];
+        double px, py;
+        if (q <= n + 1) {                       // x-line crossing (:162-164)
+            const double xg = gridc(q, half);
+            px = xg;
+            py = r.b * ((xg - r.x0th) / r.a) + r.y0th;
+        } else {                                // y-line crossing (:167-169)
+            const double yg = gridc(q - (n + 1), half);
+            px = r.a * ((yg - r.y0th) / r.b) + r.x0th;
+            py = yg;
+        }
+        s.template push<FILL, E>(px, py, r.excl, e);
+    }
+    if (s.bad_col) *bad_col = true;
+    return s.cnt;
+}
+
+template <int BEAM, bool FILL>
+__global__ void __launch_bounds__(64) k_exact_lds(RayArgs g, const int32_t *irr_list, int32_t *counts,
+                                                  const int32_t *IA, int32_t *JA, double *vA, int32_t *err) {
+    extern __shared__ __align__(16) unsigned char exact_lds[];
+    const int64_t ray = irr_list[blockIdx.x];
+    Emit e{nullptr, nullptr, 0};
+    if (FILL) {
+        const int32_t off = IA[ray];
+        e = Emit{JA + off, vA + off, IA[ray + 1] - off};
+    }
+    bool bad = false;
+    const int c = exact_ray_lds<BEAM, FILL, Emit>(g, ray, exact_lds, e, &bad);
+    if (threadIdx.x != 0) return;
+    if (FILL) {
+        if (bad) atomicOr(err, 1);
+        if (c != IA[ray + 1] - IA[ray]) atomicOr(err, 2);
+    } else {
+        counts[ray] = c;
+    }
 }
 
 // Irregular rays: the reference algorithm verbatim (t arrays, Hoare quicksort with middle
@@ -508,9 +588,15 @@ void trace(const fair_geom &geo, const std::vector<double> &ang, const std::vect
     FAIR_HIP(hipStreamSynchronize(stream()));
 
     const int nn = 2 * geo.n + 2;
+    // irregular rays: the sort in LDS when its arrays fit, else in global scratch
+    const size_t lds_bytes = (size_t)EXACT_LDS_PER_CROSSING * nn;
+    const bool in_lds = lds_bytes <= (size_t)lds_per_block();
     const size_t per_ray = ((size_t)3 * nn * sizeof(double) + (size_t)3 * nn * sizeof(int32_t) + 255) & ~(size_t)255;
-    DBuf<char> scratch(nirr > 0 ? per_ray * (size_t)nirr : 1);
-    if (nirr > 0)
+    DBuf<char> scratch(nirr > 0 && !in_lds ? per_ray * (size_t)nirr : 1);
+    if (nirr > 0 && in_lds)
+        hipLaunchKernelGGL((k_exact_lds<BEAM, false>), dim3((unsigned)nirr), dim3(64), lds_bytes, stream(), g,
+                           irr_list.p, counts.p, nullptr, nullptr, nullptr, err.p);
+    else if (nirr > 0)
         hipLaunchKernelGGL((k_exact<BEAM, false>), dim3(grid_for(nirr, 64)), dim3(64), 0, stream(),
                            g, irr_list.p, nirr, scratch.p, per_ray, counts.p, nullptr, nullptr,
                            nullptr, err.p);
@@ -524,7 +610,10 @@ void trace(const fair_geom &geo, const std::vector<double> &ang, const std::vect
     if (m > 0)
         hipLaunchKernelGGL(k_fill<BEAM>, dim3(grid_for(m, 256)), dim3(256), 0, stream(), g, m,
                            A.IA.p, irregular.p, A.JA.p, A.vA.p, err.p);
-    if (nirr > 0)
+    if (nirr > 0 && in_lds)
+        hipLaunchKernelGGL((k_exact_lds<BEAM, true>), dim3((unsigned)nirr), dim3(64), lds_bytes, stream(), g,
+                           irr_list.p, nullptr, A.IA.p, A.JA.p, A.vA.p, err.p);
+    else if (nirr > 0)
         hipLaunchKernelGGL((k_exact<BEAM, true>), dim3(grid_for(nirr, 64)), dim3(64), 0, stream(),
                            g, irr_list.p, nirr, scratch.p, per_ray, nullptr, A.IA.p, A.JA.p,
                            A.vA.p, err.p);
