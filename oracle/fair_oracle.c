@@ -603,6 +603,164 @@ int64_t oracle_art(int m, int n, const int32_t *IA, const int32_t *JA, const dou
     return q;
 }
 
+/* twin_art, AIRMs.f90:371-811: two Kaczmarz chains, x over the row order and y over the
+ * reversed order (forward + reverse pairs for symkaczmarz), with the 'errorgauge' stop (stoprule
+ * 0: slack iterations without a new minimum of ||x - y||, result (x + y)/2 at the minimum) or
+ * the 'mutualstep' stop (stoprule 1: a 2x2 least-squares step along both chains' last steps,
+ * exit when |alpha| + |beta| < 1e-5).  Uniform draws are consumed in the reference's order:
+ * scramble (2*nord) at set-up and every iteration, one per row step with rowprobability.
+ * Xstore / Ystore: n x nk (column k at Kuse(k)), EG: maxval(Kin) entries.  Returns the draws
+ * consumed, -1 for odd Kin with symkaczmarz, -2 when u runs out, -3 for x0, y0 too close. */
+static double nrm2_diff(const double *a, const double *b, int n) {
+    double s = 0.0;
+    for (int j = 0; j < n; j++) { double d = a[j] - b[j]; s += d * d; }
+    return sqrt(s);
+}
+
+static double dot_n(const double *a, const double *b, int n) {
+    double s = 0.0;
+    for (int j = 0; j < n; j++) s += a[j] * b[j];
+    return s;
+}
+
+typedef struct {
+    const int32_t *IA, *JA;
+    const double *vA, *b, *nrA, *cumul, *u;
+    int m, nord, isrand, rowprob, sym, upb, lwrb;
+    double omg, ub, lb;
+    int64_t q, nu;
+    int32_t *rows;
+} TwinCtx;
+
+/* basic_sweep (AIRMs.f90:813-858) over ord (reversed if rev); -2 when the draws run out */
+static int twin_sweep(TwinCtx *c, const int32_t *ord, int rev, double *x) {
+    for (int ii = 0; ii < c->nord; ii++) {
+        if (c->isrand && c->rowprob) {
+            if (c->q >= c->nu) return -2;
+            c->rows[ii] = oracle_rowprob_pick(c->m, c->cumul, c->u[c->q++]);
+        } else {
+            c->rows[ii] = rev ? ord[c->nord - 1 - ii] : ord[ii];
+        }
+    }
+    oracle_art_steps(c->IA, c->JA, c->vA, c->b, c->nrA, c->rows, c->nord, c->omg, c->upb, c->lwrb, c->ub, c->lb, x);
+    return 0;
+}
+
+/* the x chain's sweeps of one iteration (x: ord then reversed), or the y chain's (y: reversed
+ * then ord) */
+static int twin_chain(TwinCtx *c, const int32_t *ord, int is_y, double *v) {
+    if (twin_sweep(c, ord, is_y, v) < 0) return -2;
+    if (c->sym && twin_sweep(c, ord, !is_y, v) < 0) return -2;
+    return 0;
+}
+
+int64_t oracle_twin_art(int m, int n, const int32_t *IA, const int32_t *JA, const double *vA,
+                        const double *b, int method, int stoprule, int nk, const int32_t *Kin,
+                        const double *x0, const double *y0, const int32_t *order, int nord_in,
+                        int slack, int upb, int lwrb, double ub, double lb, double omg, double dmp,
+                        int rowprob, const double *u, int64_t nu, double *xout, int *final_it,
+                        double *Xstore, double *Ystore, double *EG) {
+    int isrand = (method == 2);
+    int32_t *Kuse = malloc(sizeof(int32_t) * nk);
+    int maxK = 0;
+    for (int k = 0; k < nk; k++) { Kuse[k] = Kin[k]; if (Kin[k] > maxK) maxK = Kin[k]; }
+    if (method == 1) {
+        for (int k = 0; k < nk; k++) if (Kin[k] % 2 == 1) { free(Kuse); return -1; }
+        for (int k = 0; k < nk; k++) Kuse[k] = Kuse[k] / 2;
+    }
+    for (int k = 0; k < maxK; k++) EG[k] = 0.0;
+    int32_t *ord = malloc(sizeof(int32_t) * (m + 1)), *tmp = malloc(sizeof(int32_t) * (m + 1));
+    int nord;
+    if (order) { nord = nord_in; for (int i = 0; i < nord; i++) ord[i] = order[i]; }
+    else { nord = m; for (int i = 0; i < m; i++) ord[i] = i + 1; }
+    double *nrA = malloc(sizeof(double) * m), *cumul = malloc(sizeof(double) * m);
+    oracle_row_norms(m, IA, vA, 2, nrA);                                  /* :604-606 */
+    double mx = -INFINITY;
+    for (int i = 0; i < m; i++) nrA[i] = nrA[i] * nrA[i];
+    for (int i = 0; i < m; i++) if (nrA[i] > mx) mx = nrA[i];
+    for (int i = 0; i < m; i++) nrA[i] = nrA[i] + dmp * mx;
+    int l = nord; nord = 0;                                               /* :609-617 */
+    for (int i = 0; i < l; i++) if (nrA[ord[i] - 1] > 0) tmp[nord++] = ord[i];
+    for (int i = 0; i < nord; i++) ord[i] = tmp[i];
+    TwinCtx c = {IA, JA, vA, b, nrA, cumul, u, m, nord, isrand, rowprob, method == 1, upb, lwrb, omg, ub, lb,
+                 0, nu, malloc(sizeof(int32_t) * (nord > 0 ? nord : 1))};
+    int64_t ret = 0;
+    double *x = malloc(sizeof(double) * n), *y = malloc(sizeof(double) * n);
+    double *v = malloc(sizeof(double) * n), *w = malloc(sizeof(double) * n);
+    for (int j = 0; j < n; j++) v[j] = w[j] = 0.0;
+    if (isrand) {                                                        /* :620-633 */
+        if (c.q + 2 * (int64_t)nord > nu) { ret = -2; goto done; }
+        oracle_scramble(nord, u + c.q, ord); c.q += 2 * (int64_t)nord;
+        if (rowprob) oracle_rowprob_cumul(m, nord, nrA, cumul);
+    }
+    int startiter;
+    if (x0 && y0) {                                                      /* :650-654 */
+        for (int j = 0; j < n; j++) { x[j] = x0[j]; y[j] = y0[j]; }
+        if (nrm2_diff(x0, y0, n) < 1e-10) { ret = -3; goto done; }
+        startiter = 1;
+    } else {                                                             /* :634-649, 655-666 */
+        const double *s0 = x0 ? x0 : y0;
+        for (int j = 0; j < n; j++) x[j] = y[j] = s0 ? s0[j] : 0.0;
+        if (twin_chain(&c, ord, 0, x) < 0 || twin_chain(&c, ord, 1, y) < 0) { ret = -2; goto done; }
+        EG[0] = nrm2_diff(x, y, n);
+        startiter = 2;
+    }
+    int maxiter = 0;
+    for (int k = 0; k < nk; k++) if (Kuse[k] > maxiter) maxiter = Kuse[k];
+    int rec = 0, trck = 0, iter;
+    double minEG = HUGE_VAL;
+    for (iter = startiter; iter <= maxiter; iter++) {                    /* :672-770 */
+        if (isrand) {
+            if (c.q + 2 * (int64_t)nord > nu) { ret = -2; goto done; }
+            oracle_scramble(nord, u + c.q, ord); c.q += 2 * (int64_t)nord;
+        }
+        if (stoprule == 1) memcpy(v, x, sizeof(double) * n);
+        if (twin_chain(&c, ord, 0, x) < 0) { ret = -2; goto done; }
+        if (stoprule == 1) for (int j = 0; j < n; j++) { v[j] = x[j] - v[j]; x[j] = x[j] - v[j]; }
+        if (stoprule == 1) memcpy(w, y, sizeof(double) * n);
+        if (twin_chain(&c, ord, 1, y) < 0) { ret = -2; goto done; }
+        if (stoprule == 1) for (int j = 0; j < n; j++) { w[j] = y[j] - w[j]; y[j] = y[j] - w[j]; }
+        if (stoprule == 0) {
+            EG[iter - 1] = nrm2_diff(x, y, n);
+            if (EG[iter - 1] < minEG) {
+                minEG = EG[iter - 1];
+                memcpy(v, x, sizeof(double) * n);
+                memcpy(w, y, sizeof(double) * n);
+                trck = 0;
+                *final_it = iter;
+            } else {
+                trck = trck + 1;
+                if (trck >= slack) break;
+            }
+        } else {
+            double nw = sqrt(dot_n(w, w, n)), nv = sqrt(dot_n(v, v, n));
+            for (int j = 0; j < n; j++) { w[j] = w[j] / nw; v[j] = v[j] / nv; }
+            double m1 = dot_n(v, v, n), m2 = dot_n(w, w, n), m3 = -dot_n(v, w, n);
+            double r1 = 0.0, r2 = 0.0;
+            for (int j = 0; j < n; j++) { double d = x[j] - y[j]; r1 += v[j] * d; r2 += w[j] * d; }
+            r1 = -r1;
+            double det = m1 * m2 - m3 * m3;
+            double alph = (r1 * m2 - r2 * m3) / det;
+            double bet = (m1 * r2 - m3 * r1) / det;
+            for (int j = 0; j < n; j++) { x[j] = x[j] + alph * v[j]; y[j] = y[j] + bet * w[j]; }
+            EG[iter - 1] = nrm2_diff(x, y, n);
+            if (fabs(alph) + fabs(bet) < 1e-5) break;
+        }
+        if (rec < nk && iter == Kuse[rec]) {
+            memcpy(Xstore + (int64_t)rec * n, x, sizeof(double) * n);
+            memcpy(Ystore + (int64_t)rec * n, y, sizeof(double) * n);
+            rec++;
+        }
+    }
+    if (stoprule == 0) { for (int j = 0; j < n; j++) xout[j] = .5 * (v[j] + w[j]); }
+    else { for (int j = 0; j < n; j++) xout[j] = .5 * (x[j] + y[j]); *final_it = iter; }
+    ret = c.q;
+done:
+    free(Kuse); free(ord); free(tmp); free(nrA); free(cumul); free(c.rows);
+    free(x); free(y); free(v); free(w);
+    return ret;
+}
+
 /* Number of non-empty rows (the `nord` of art for the natural order), used to size the
  * uniform stream. */
 int oracle_count_nonempty(int m, const int32_t *IA) {

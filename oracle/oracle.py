@@ -79,6 +79,12 @@ class Oracle:
                                  C.c_int64, _F64P]
         L.oracle_count_nonempty.restype = C.c_int
         L.oracle_count_nonempty.argtypes = [C.c_int, _I32P]
+        L.oracle_twin_art.restype = C.c_int64
+        L.oracle_twin_art.argtypes = [C.c_int, C.c_int, _I32P, _I32P, _F64P, _F64P, C.c_int, C.c_int,
+                                      C.c_int, _I32P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, _F64P, C.c_int64, _F64P,
+                                      C.POINTER(C.c_int), _F64P, _F64P, _F64P]
 
     # -- system_generation -------------------------------------------------------------
     def linspace(self, a, b, n):
@@ -198,6 +204,33 @@ class Oracle:
             raise ValueError("too few uniforms supplied")
         return out, q
 
+    def twin_art(self, method, stoprule, IA, JA, vA, n, b, kin, u=None, x0=None, y0=None, order=None,
+                 slack=7, lb=None, ub=None, relax=1.0, damp=0.0, rowprob=False):
+        """twin_art: returns (xout, final_it, Xstore[nk, n], Ystore[nk, n], EG, draws consumed)."""
+        kin = np.ascontiguousarray(np.atleast_1d(kin), np.int32)
+        nk = len(kin)
+        xout = np.zeros(n)
+        Xs, Ys = np.zeros((nk, n)), np.zeros((nk, n))
+        EG = np.zeros(int(kin.max()))
+        fi = C.c_int(0)
+        x0c = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        y0c = None if y0 is None else np.ascontiguousarray(y0, np.float64)
+        oc = None if order is None else np.ascontiguousarray(order, np.int32)
+        u = np.zeros(1) if u is None else np.ascontiguousarray(u, np.float64)
+        q = self.L.oracle_twin_art(len(IA) - 1, n, IA, _fix32(JA), _fix64(vA), np.ascontiguousarray(b, np.float64),
+                                   {"kaczmarz": 0, "symkaczmarz": 1, "randkaczmarz": 2}[method],
+                                   {"errorgauge": 0, "mutualstep": 1}[stoprule], nk, kin, _vp(x0c), _vp(y0c),
+                                   _vp(oc), 0 if oc is None else len(oc), slack, ub is not None, lb is not None,
+                                   0.0 if ub is None else ub, 1.0 if lb is None else lb, relax, damp, bool(rowprob),
+                                   u, len(u), xout, C.byref(fi), Xs, Ys, EG)
+        if q == -1:
+            raise ValueError("Kin should all be multiples of 2.")
+        if q == -2:
+            raise ValueError("too few uniforms supplied")
+        if q == -3:
+            raise ValueError("Initial conditions x0 and y0 too close.")
+        return xout, fi.value, Xs, Ys, EG, q
+
     def count_nonempty(self, IA):
         return self.L.oracle_count_nonempty(len(IA) - 1, IA)
 
@@ -257,6 +290,8 @@ class Reference:
         L.ref_normest.argtypes = [i, d, i]
         L.ref_art.argtypes = [i, i, i, _F64P, i, _I32P, i, i, _F64P, i, i, _I32P, i, i, d, d, d, d,
                               i, _F64P]
+        L.ref_twin_art.argtypes = [i, i, i, i, _F64P, i, _I32P, i, i, i, _F64P, _F64P, i, i, i, d, d, d, d, i,
+                                   _F64P, C.POINTER(i), _F64P, _F64P, _F64P, i]
         L.ref_rng_seed.argtypes = [i]
         L.ref_rng_draw.argtypes = [i, _F64P]
         L.ref_shepplogan.argtypes = [i, _F64P]
@@ -337,6 +372,26 @@ class Reference:
                       rho, tol, ub is not None, lb is not None, 0.0 if ub is None else ub,
                       1.0 if lb is None else lb, out)
         return out
+
+    def twin_art(self, slot, method, stoprule, b, kin, x0=None, y0=None, slack=7, lb=None, ub=None, relax=1.0,
+                 damp=0.0, rowprob=False):
+        """Returns (xout, final_it, Xstore[nk, n], Ystore[nk, n], EG)."""
+        m, n, _ = self.info(slot)
+        kin = np.ascontiguousarray(np.atleast_1d(kin), np.int32)
+        nk = len(kin)
+        xout = np.zeros(n)
+        Xs, Ys = np.zeros((nk, n)), np.zeros((nk, n))
+        neg = int(kin.max())
+        EG = np.zeros(neg)
+        fi = C.c_int(0)
+        x0a = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, np.float64)
+        y0a = np.zeros(n) if y0 is None else np.ascontiguousarray(y0, np.float64)
+        self.L.ref_twin_art(slot, {"kaczmarz": 0, "symkaczmarz": 1, "randkaczmarz": 2}[method],
+                            {"errorgauge": 0, "mutualstep": 1}[stoprule], m, np.ascontiguousarray(b, np.float64),
+                            nk, kin, x0 is not None, y0 is not None, n, x0a, y0a, slack, ub is not None,
+                            lb is not None, 0.0 if ub is None else ub, 1.0 if lb is None else lb, relax, damp,
+                            bool(rowprob), xout, C.byref(fi), Xs, Ys, EG, neg)
+        return xout, fi.value, Xs, Ys, EG
 
     def normest(self, slot, tol=1e-5, maxit=100):
         return self.L.ref_normest(slot, tol, maxit)

@@ -269,6 +269,56 @@ contains
         xout = xo
     end subroutine ref_art
 
+    ! twin_art (AIRMs.f90:371-811); stoprule 0 = 'errorgauge', 1 = 'mutualstep'.
+    subroutine ref_twin_art(slot, method, stoprule, m, b, nk, kin, hasx0, hasy0, n, x0, y0, slack, &
+                            upb, lwrb, ub, lb, relax, damp, rowprob, xout, final_it, xs, ys, eg, neg) &
+        bind(C, name='ref_twin_art')
+        integer(c_int), value :: slot, method, stoprule, m, nk, hasx0, hasy0, n, slack, upb, lwrb, rowprob, neg
+        real(c_double), value :: ub, lb, relax, damp
+        real(c_double), intent(in) :: b(m), x0(n), y0(n)
+        integer(c_int), intent(in) :: kin(nk)
+        real(c_double), intent(out) :: xout(n), xs(n, nk), ys(n, nk), eg(neg)
+        integer(c_int), intent(out) :: final_it
+        type(opts) :: options
+        character(len=12) :: meth, rule
+        real(kind=8) :: xo(n), xst(n, nk), yst(n, nk), egv(maxval(kin))
+        integer :: fi, sl
+        call fill_opts(options, upb, lwrb, ub, lb, relax, damp, rowprob)
+        select case (method)
+        case (0)
+            meth = 'kaczmarz'
+        case (1)
+            meth = 'symkaczmarz'
+        case default
+            meth = 'randkaczmarz'
+        end select
+        rule = 'errorgauge'
+        if (stoprule == 1) rule = 'mutualstep'
+        xst = 0d0
+        yst = 0d0
+        egv = 0d0
+        fi = 0
+        sl = slack
+        if (hasx0 /= 0 .and. hasy0 /= 0) then
+            call twin_art(xo, fi, xst, yst, egv, trim(meth), trim(rule), mats(slot), b, kin, x0=x0, y0=y0, &
+                          slack=sl, options=options)
+        else if (hasx0 /= 0) then
+            call twin_art(xo, fi, xst, yst, egv, trim(meth), trim(rule), mats(slot), b, kin, x0=x0, &
+                          slack=sl, options=options)
+        else if (hasy0 /= 0) then
+            call twin_art(xo, fi, xst, yst, egv, trim(meth), trim(rule), mats(slot), b, kin, y0=y0, &
+                          slack=sl, options=options)
+        else
+            call twin_art(xo, fi, xst, yst, egv, trim(meth), trim(rule), mats(slot), b, kin, slack=sl, &
+                          options=options)
+        end if
+        xout = xo
+        final_it = fi
+        xs = xst
+        ys = yst
+        eg = egv(1:neg)
+    end subroutine ref_twin_art
+
     ! Seed the runtime generator deterministically from one integer.
     subroutine ref_rng_seed(seed) bind(C, name='ref_rng_seed')
         integer(c_int), value :: seed

@@ -13,8 +13,10 @@ Outputs (numpy .npz, inputs and expected outputs only):
   art_small.npz      Kaczmarz family on a small fan problem, with the uniform streams drawn
   c2.npz             (--c2) the N=512 problem: matrix hashes + sampled rows, the hash of b, SART
                      and Cimmino 200-iteration iterates
+  twin_small.npz     (--twin) twin_art with both stopping rules on a small fan problem, with the
+                     uniform streams drawn
 
-usage: python tests/golden/make_golden.py [--c2]
+usage: python tests/golden/make_golden.py [--c2 | --twin]
 """
 from __future__ import annotations
 
@@ -161,6 +163,48 @@ def write_art(ref):
     np.savez_compressed(os.path.join(HERE, "art_small.npz"), **d)
 
 
+def write_twin(ref):
+    g = cf.Geometry(n=16, p=23, nth=30, deg_start=0.37, deg_step=12.0)
+    IA, JA, vA, xt, b = problem(ref, g, cf.shepplogan, 0.02, 7)
+    n, m = g.ncols, g.m
+    d = {"IA": IA, "JA": JA, "vA": vA, "b": b, "xt": xt}
+    x0 = np.random.default_rng(8).random(n)
+    cases = [("kaczmarz", "errorgauge", [10, 20, 40], False, {"lb": 0.0, "ub": 1.0}),
+             ("kaczmarz", "mutualstep", [10, 20, 40], False, {"lb": 0.0, "ub": 1.0}),
+             ("symkaczmarz", "errorgauge", [10, 40], False, {"lb": 0.0, "ub": 1.0, "slack": 3}),
+             ("symkaczmarz", "mutualstep", [10, 20], False, {}),
+             ("randkaczmarz", "errorgauge", [10, 30], False, {"lb": 0.0, "ub": 1.0}),
+             ("randkaczmarz", "mutualstep", [10, 30], True, {"lb": 0.0, "ub": 1.0}),
+             ("randkaczmarz", "errorgauge", [20], True, {"ub": 1.0, "relax": 0.8, "damp": 0.01}),
+             ("kaczmarz", "errorgauge", [15], False, {"lb": 0.0, "ub": 1.0, "x0": x0}),
+             ("kaczmarz", "mutualstep", [15], False, {"lb": 0.0, "x0": x0, "y0": np.zeros(n)})]
+    for i, (meth, rule, kin, rp, kw) in enumerate(cases):
+        ref.set(0, IA, JA, vA, n)
+        ref.rng_seed(200 + i)
+        xo, fi, Xs, Ys, EG = ref.twin_art(0, meth, rule, b, kin, rowprob=rp, **kw)
+        ref.rng_seed(200 + i)
+        u = ref.rng_draw(4 * m * (max(kin) + 2)) if meth == "randkaczmarz" else np.zeros(1)
+        if meth == "randkaczmarz":       # keep the draws the run consumed (counted by the oracle)
+            from oracle.oracle import Oracle
+            q = Oracle().twin_art(meth, rule, IA, JA, vA, n, b, kin, u=u, rowprob=rp, **kw)[5]
+            u = u[:q]
+        d[f"case{i}/method"] = np.array(meth)
+        d[f"case{i}/stoprule"] = np.array(rule)
+        d[f"case{i}/kin"] = np.array(kin, np.int32)
+        d[f"case{i}/rowprob"] = np.array(rp)
+        for key, dflt in (("lb", np.nan), ("ub", np.nan), ("relax", 1.0), ("damp", 0.0), ("slack", 7)):
+            d[f"case{i}/{key}"] = np.array(kw.get(key, dflt))
+        d[f"case{i}/x0"] = kw["x0"] if "x0" in kw else np.zeros(0)
+        d[f"case{i}/y0"] = kw["y0"] if "y0" in kw else np.zeros(0)
+        d[f"case{i}/u"] = u
+        d[f"case{i}/xout"] = xo
+        d[f"case{i}/final_it"] = np.array(fi, np.int32)
+        d[f"case{i}/Xstore"] = Xs
+        d[f"case{i}/Ystore"] = Ys
+        d[f"case{i}/EG"] = EG
+    np.savez_compressed(os.path.join(HERE, "twin_small.npz"), **d)
+
+
 def write_c2(ref):
     g = cf.C2
     IA, JA, vA, xt, b = problem(ref, g, cf.grains, 0.005, 2)
@@ -180,6 +224,9 @@ def main():
     ref = Reference()
     if "--c2" in sys.argv:
         write_c2(ref)
+        return
+    if "--twin" in sys.argv:
+        write_twin(ref)
         return
     write_small(ref)
     write_c1(ref)

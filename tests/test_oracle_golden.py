@@ -101,6 +101,40 @@ def test_art_family(oracle):
     assert bitequal(X, g["order_X"])
 
 
+def twin_case_kwargs(g, c):
+    lb, ub = float(g[f"{c}/lb"]), float(g[f"{c}/ub"])
+    x0, y0 = g[f"{c}/x0"], g[f"{c}/y0"]
+    return dict(u=g[f"{c}/u"], x0=x0 if len(x0) else None, y0=y0 if len(y0) else None,
+                slack=int(g[f"{c}/slack"]), lb=lb if np.isfinite(lb) else None, ub=ub if np.isfinite(ub) else None,
+                relax=float(g[f"{c}/relax"]), damp=float(g[f"{c}/damp"]), rowprob=bool(g[f"{c}/rowprob"]))
+
+
+def test_twin_art(oracle):
+    """twin_art (AIRMs.f90:371-811) against the compiled reference: with the error-gauge rule the
+    iterates are bit-exact and the gauge (NORM2, whose last bit is the compiler runtime's) within
+    1e-14; the mutual-step rule normalises with NORM2 and solves a 2x2 system from DOT_PRODUCTs,
+    so its iterates carry those last bits (bar 1e-12)."""
+    g = load_golden("twin_small.npz")
+    IA, JA, vA, b, n = g["IA"], g["JA"], g["vA"], g["b"], len(g["xt"])
+    cases = sorted({k.split("/")[0] for k in g.files if k.startswith("case")})
+    assert len(cases) == 9
+    for c in cases:
+        rule = str(g[f"{c}/stoprule"])
+        xo, fi, Xs, Ys, EG, q = oracle.twin_art(str(g[f"{c}/method"]), rule, IA, JA, vA, n, b, g[f"{c}/kin"],
+                                                **twin_case_kwargs(g, c))
+        assert fi == int(g[f"{c}/final_it"]), c
+        if str(g[f"{c}/method"]) == "randkaczmarz":
+            assert q == len(g[f"{c}/u"]), c
+        if rule == "errorgauge":
+            assert bitequal(xo, g[f"{c}/xout"]), c
+            assert bitequal(Xs, g[f"{c}/Xstore"]) and bitequal(Ys, g[f"{c}/Ystore"]), c
+            assert np.max(np.abs(EG - g[f"{c}/EG"])) <= 1e-14 * np.max(np.abs(g[f"{c}/EG"])), c
+        else:
+            ref = g[f"{c}/xout"]
+            assert np.linalg.norm(xo - ref) <= 1e-12 * np.linalg.norm(ref), c
+            assert np.max(np.abs(EG - g[f"{c}/EG"])) <= 1e-12 * np.max(np.abs(g[f"{c}/EG"])), c
+
+
 def test_c2_matrix(oracle):
     from fortran_airtools_amd import configs as cf
     gold = load_golden("c2.npz")
