@@ -189,6 +189,42 @@ def art(art_method: str, A: CSRMatrix, b, Kin, x0=None, order=None, options: Opt
     return out
 
 
+STOP_RULES = {"errorgauge": 0, "mutualstep": 1}
+
+
+def twin_art(art_method: str, stoprule: str, A: CSRMatrix, b, Kin, x0=None, y0=None, order=None,
+             slack: int = 7, options: Opts | None = None, uniforms=None, seed: int = 0):
+    """call twin_art(xout, final_it, Xstore, Ystore, EG, art_method, stoprule, A, b, Kin, x0, y0,
+    order, slack, options), AIRMs.f90:371-811.  Returns (xout, final_it, Xstore, Ystore, EG) with
+    Xstore / Ystore of shape (size(Kin), n).  `uniforms`: the random_number draws of randomized
+    Kaczmarz in the reference's order (seeded draws when not supplied)."""
+    if stoprule not in STOP_RULES:
+        raise FairError(FAIR_ERR_METHOD, f"Unknown stopping rule {stoprule}. Use errorgauge or mutualstep.")
+    meth, K, ordv = _art_args(art_method, Kin, order)
+    b = _f64(b, A.m)
+    x0a = None if x0 is None else _f64(x0, A.n)
+    y0a = None if y0 is None else _f64(y0, A.n)
+    o = _opts(options)
+    nord = 0 if ordv is None else ordv.shape[0]
+    if uniforms is None and meth == 2:
+        cnt = C.c_int64()
+        check(lib().fair_twin_art_uniform_bound(A.handle, meth, K.shape[0], _ptr(K), _ptr(ordv), nord, C.byref(o),
+                                                C.byref(cnt)))
+        uniforms = np.random.default_rng(seed).random(cnt.value)
+    u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float64)
+    xout = np.zeros(A.n)
+    Xs = np.zeros((K.shape[0], A.n))
+    Ys = np.zeros((K.shape[0], A.n))
+    EG = np.zeros(int(K.max()))
+    fi = C.c_int32()
+    used = C.c_int64()
+    check(lib().fair_twin_art(A.handle, meth, STOP_RULES[stoprule], _ptr(b), K.shape[0], _ptr(K), _ptr(x0a),
+                              _ptr(y0a), _ptr(ordv), nord, int(slack), C.byref(o), _ptr(u),
+                              0 if u is None else u.shape[0], _ptr(xout), C.byref(fi), _ptr(Xs), _ptr(Ys), _ptr(EG),
+                              C.byref(used)))
+    return xout, fi.value, Xs, Ys, EG
+
+
 class SirtGroup:
     """Independent sirt problems (each its own CSRMatrix, sizes may differ) iterated together:
     two launches per iteration for all of them, each iterate bit-identical to its own sirt run.

@@ -126,6 +126,46 @@ struct SqDiff {
         return d * d;
     }
 };
+struct DotOf {
+    const double *a, *b;
+    __device__ __forceinline__ double operator()(int64_t i) const { return a[i] * b[i]; }
+};
+// v . (x - y), the difference formed first as the reference's array temporary does
+struct DotDiff {
+    const double *v, *x, *y;
+    __device__ __forceinline__ double operator()(int64_t i) const { return v[i] * (x[i] - y[i]); }
+};
+
+// twin_art's mutual-step bookkeeping (AIRMs.f90:694-705): d = new - old; new = new - d
+__global__ void k_step_back(double *x, double *v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    v[i] = x[i] - v[i];
+    x[i] = x[i] - v[i];
+}
+
+// a = a / na ; b = b / nb (:716-717)
+__global__ void k_div2(double *a, double na, double *b, double nb, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    b[i] = b[i] / nb;
+    a[i] = a[i] / na;
+}
+
+// x = x + alpha v ; y = y + beta w (:726-727)
+__global__ void k_axpy2(double *x, const double *v, double alpha, double *y, const double *w, double beta,
+                        int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = x[i] + alpha * v[i];
+    y[i] = y[i] + beta * w[i];
+}
+
+// out = .5 * (a + b) (:791-795)
+__global__ void k_half_sum(const double *a, const double *b, double *out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = .5 * (a[i] + b[i]);
+}
 
 template <class F>
 double dev_sum(int64_t n, F f, DBuf<double> &part, DBuf<double> &out) {
@@ -239,15 +279,25 @@ __device__ __forceinline__ ArtRow art_row(const int32_t *rows, int64_t s, int64_
     return d;
 }
 
-__global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(const int32_t *__restrict__ rows, int64_t nsteps,
-                                                             const int32_t *__restrict__ IA,
+// Up to two independent chains (row sequence, iterate), one workgroup each: twin_art sweeps its
+// x and y chains in the same launch.
+struct SweepChains {
+    const int32_t *rows[2];
+    int64_t nsteps[2];
+    double *x[2];
+};
+
+__global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, const int32_t *__restrict__ IA,
                                                              const int32_t *__restrict__ JA,
                                                              const double *__restrict__ vA,
                                                              const double *__restrict__ b,
                                                              const double *__restrict__ nrA,
                                                              int32_t ncols, double relax, int lwrb,
-                                                             double lb, int upb, double ub, double *x) {
+                                                             double lb, int upb, double ub) {
     const int lane = threadIdx.x;
+    const int32_t *__restrict__ rows = ch.rows[blockIdx.x];
+    const int64_t nsteps = ch.nsteps[blockIdx.x];
+    double *x = ch.x[blockIdx.x];
     if (nsteps <= 0) return;
     ArtRow cur = art_row(rows, 0, nsteps, IA, b, nrA);
     ArtRow nxt = art_row(rows, 1, nsteps, IA, b, nrA);
@@ -621,6 +671,20 @@ extern "C" int fair_art_uniform_count(fair_csr *h, int32_t method, int32_t nk, c
     });
 }
 
+// Upper bound of the draws fair_twin_art can consume (it stops early by its rule).
+extern "C" int fair_twin_art_uniform_bound(fair_csr *h, int32_t method, int32_t nk, const int32_t *Kin,
+                                           const int32_t *order, int32_t nord, const fair_opts *opts,
+                                           int64_t *count) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(count != nullptr, FAIR_ERR_ARG, "null argument");
+        DBuf<double> d_nrA;
+        ArtPlan P = plan_art(h, method, nk, Kin, order, nord, opts, d_nrA);
+        *count = P.isrand ? ((int64_t)P.maxiter + 1) * (2 * (int64_t)P.nord + (P.rowprob ? 2 * (int64_t)P.nord : 0))
+                          : 0;
+    });
+}
+
 extern "C" int fair_art_steps_dev(fair_csr *h, const int32_t *d_rows, int64_t nsteps, const double *d_b,
                                   const double *d_nrA, const fair_opts *o, double *d_x) {
     return guard([&] {
@@ -628,10 +692,173 @@ extern "C" int fair_art_steps_dev(fair_csr *h, const int32_t *d_rows, int64_t ns
         const Box box = make_box(o);
         const double relax = o ? o->relaxpar : 1.0;
         if (nsteps > 0)
-            hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(), d_rows, nsteps,
-                               h->A.IA.p, h->A.JA.p, h->A.vA.p, d_b, d_nrA, h->A.n, relax, box.lwrb,
-                               box.lb, box.upb, box.ub, d_x);
+            hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
+                               SweepChains{{d_rows, nullptr}, {nsteps, 0}, {d_x, nullptr}}, h->A.IA.p, h->A.JA.p,
+                               h->A.vA.p, d_b, d_nrA, h->A.n, relax, box.lwrb, box.lb, box.upb, box.ub);
         launch_check("art sweep");
+    });
+}
+
+namespace {
+
+// Row sequence of one basic_sweep call (AIRMs.f90:813-858): ord forward or reversed, or, with
+// rowprobability, one draw per step searched in the cumulative table (first ri with cumul(ri) >= u).
+int64_t sweep_rows(const ArtPlan &P, const std::vector<int32_t> &ord, bool rev, const double *u, int64_t nu,
+                   int64_t q, int32_t m, int32_t *out) {
+    for (int ii = 0; ii < P.nord; ++ii) {
+        if (P.rowprob) {
+            FAIR_CHECK(q < nu, FAIR_ERR_ARG, "too few uniforms for twin_art");
+            auto it = std::lower_bound(P.cumul.begin(), P.cumul.end(), u[q++]);
+            out[ii] = std::min((int)(it - P.cumul.begin()) + 1, (int)m);
+        } else {
+            out[ii] = rev ? ord[P.nord - 1 - ii] : ord[ii];
+        }
+    }
+    return q;
+}
+
+}  // namespace
+
+// twin_art, AIRMs.f90:371-811: chains x (row order, plus its reverse for symkaczmarz) and y
+// (reversed order, plus the forward one) run as two workgroups of one sweep launch per
+// iteration; stoprule 0 = 'errorgauge' (stop after `slack` iterations without a new minimum of
+// ||x - y||, result (x + y)/2 at the minimum), 1 = 'mutualstep' (2x2 least-squares step along
+// both chains' last steps, stop when |alpha| + |beta| < 1e-5).  Draws as for fair_art, consumed
+// in the reference's order; *consumed receives their number.  Xstore / Ystore: nk x n (row k =
+// iterate at Kuse(k)), EG: maxval(Kin) entries (zero where not reached).
+extern "C" int fair_twin_art(fair_csr *h, int32_t method, int32_t stoprule, const double *b, int32_t nk,
+                             const int32_t *Kin, const double *x0, const double *y0, const int32_t *order,
+                             int32_t nord, int32_t slack, const fair_opts *opts, const double *uniforms, int64_t nu,
+                             double *xout, int32_t *final_it, double *Xstore, double *Ystore, double *EG,
+                             int64_t *consumed) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(b && xout && final_it && Xstore && Ystore && EG, FAIR_ERR_ARG, "null argument");
+        FAIR_CHECK(stoprule == 0 || stoprule == 1, FAIR_ERR_METHOD,
+                   "Unknown stopping rule. Use errorgauge or mutualstep.");
+        DBuf<double> d_nrA;
+        ArtPlan P = plan_art(h, method, nk, Kin, order, nord, opts, d_nrA);
+        FAIR_CHECK(!P.isrand || uniforms || nu == 0, FAIR_ERR_ARG, "null uniforms");
+        const int32_t m = h->A.m, n = h->A.n;
+        const Box box = make_box(opts);
+        const double relax = opts ? opts->relaxpar : 1.0;
+        const bool sym = P.method == FAIR_SYMKACZMARZ;
+        int maxK = 0;
+        for (int k = 0; k < nk; ++k) maxK = std::max(maxK, Kin[k]);
+        for (int k = 0; k < maxK; ++k) EG[k] = 0.0;
+        DBuf<double> db(m > 0 ? m : 1), x(n > 0 ? n : 1), y(n > 0 ? n : 1), v(n > 0 ? n : 1), w(n > 0 ? n : 1),
+            part(RED_BLOCKS), out(1);
+        db.upload(b, m);
+        fill(v.p, n, 0.0);
+        fill(w.p, n, 0.0);
+        const int per_chain = sym ? 2 * P.nord : P.nord;
+        std::vector<int32_t> sx(std::max(1, per_chain)), sy(std::max(1, per_chain));
+        DBuf<int32_t> dsx(sx.size()), dsy(sy.size());
+        std::vector<int32_t> ord = P.ord;
+        int64_t q = 0;
+        auto scramble_now = [&] {
+            FAIR_CHECK(q + 2 * (int64_t)P.nord <= nu, FAIR_ERR_ARG, "too few uniforms for twin_art");
+            scramble(P.nord, uniforms + q, ord.data());
+            q += 2 * (int64_t)P.nord;
+        };
+        const unsigned nb = grid_for(n, 256);
+        // both chains' sweeps of one iteration, draws taken in the reference's call order
+        auto sweep_both = [&] {
+            q = sweep_rows(P, ord, false, uniforms, nu, q, m, sx.data());
+            if (sym) q = sweep_rows(P, ord, true, uniforms, nu, q, m, sx.data() + P.nord);
+            q = sweep_rows(P, ord, true, uniforms, nu, q, m, sy.data());
+            if (sym) q = sweep_rows(P, ord, false, uniforms, nu, q, m, sy.data() + P.nord);
+            if (per_chain <= 0) return;
+            dsx.upload(sx.data(), per_chain);
+            dsy.upload(sy.data(), per_chain);
+            hipLaunchKernelGGL(k_art_sweep, dim3(2), dim3(SWEEP_THREADS), 0, stream(),
+                               SweepChains{{dsx.p, dsy.p}, {(int64_t)per_chain, (int64_t)per_chain}, {x.p, y.p}},
+                               h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p, n, relax, box.lwrb, box.lb, box.upb,
+                               box.ub);
+            launch_check("twin sweep");
+        };
+        auto norm_diff = [&](const double *a, const double *c) { return std::sqrt(dev_sum(n, SqDiff{a, c}, part, out)); };
+        if (P.isrand) scramble_now();                                   // :620-622
+        int startiter;
+        if (x0 && y0) {                                                 // :650-654
+            x.upload(x0, n);
+            y.upload(y0, n);
+            FAIR_CHECK(norm_diff(x.p, y.p) >= 1e-10, FAIR_ERR_ARG, "Initial conditions x0 and y0 too close.");
+            startiter = 1;
+        } else {                                                        // :634-649, 655-666
+            const double *s0 = x0 ? x0 : y0;
+            if (s0) {
+                x.upload(s0, n);
+                y.upload(s0, n);
+            } else {
+                fill(x.p, n, 0.0);
+                fill(y.p, n, 0.0);
+            }
+            sweep_both();
+            EG[0] = norm_diff(x.p, y.p);
+            startiter = 2;
+        }
+        int maxiter = 0;
+        for (int k : P.Kuse) maxiter = std::max(maxiter, k);
+        size_t rec = 0;
+        int trck = 0, iter;
+        double minEG = HUGE_VAL;
+        *final_it = 0;
+        for (iter = startiter; iter <= maxiter; ++iter) {               // :672-770
+            if (P.isrand) scramble_now();
+            if (stoprule == 1) {
+                hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, stream(), x.p, v.p, (int64_t)n);
+                hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, stream(), y.p, w.p, (int64_t)n);
+            }
+            sweep_both();
+            if (stoprule == 0) {
+                const double eg = norm_diff(x.p, y.p);
+                EG[iter - 1] = eg;
+                if (eg < minEG) {
+                    minEG = eg;
+                    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, stream(), x.p, v.p, (int64_t)n);
+                    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, stream(), y.p, w.p, (int64_t)n);
+                    trck = 0;
+                    *final_it = iter;
+                } else if (++trck >= slack) {
+                    break;
+                }
+            } else {
+                hipLaunchKernelGGL(k_step_back, dim3(nb), dim3(256), 0, stream(), x.p, v.p, (int64_t)n);
+                hipLaunchKernelGGL(k_step_back, dim3(nb), dim3(256), 0, stream(), y.p, w.p, (int64_t)n);
+                const double nw = std::sqrt(dev_sum(n, SqOf{w.p}, part, out));
+                const double nv = std::sqrt(dev_sum(n, SqOf{v.p}, part, out));
+                hipLaunchKernelGGL(k_div2, dim3(nb), dim3(256), 0, stream(), v.p, nv, w.p, nw, (int64_t)n);
+                const double m1 = dev_sum(n, DotOf{v.p, v.p}, part, out);
+                const double m2 = dev_sum(n, DotOf{w.p, w.p}, part, out);
+                const double m3 = -dev_sum(n, DotOf{v.p, w.p}, part, out);
+                const double r1 = -dev_sum(n, DotDiff{v.p, x.p, y.p}, part, out);
+                const double r2 = dev_sum(n, DotDiff{w.p, x.p, y.p}, part, out);
+                const double det = m1 * m2 - m3 * m3;
+                const double alph = (r1 * m2 - r2 * m3) / det;
+                const double bet = (m1 * r2 - m3 * r1) / det;
+                hipLaunchKernelGGL(k_axpy2, dim3(nb), dim3(256), 0, stream(), x.p, v.p, alph, y.p, w.p, bet,
+                                   (int64_t)n);
+                EG[iter - 1] = norm_diff(x.p, y.p);
+                if (std::fabs(alph) + std::fabs(bet) < 1e-5) break;
+            }
+            if (rec < P.Kuse.size() && iter == P.Kuse[rec]) {           // :761-764
+                x.download(Xstore + rec * (size_t)n, n);
+                y.download(Ystore + rec * (size_t)n, n);
+                ++rec;
+            }
+        }
+        DBuf<double> dout(n > 0 ? n : 1);
+        if (stoprule == 0) {
+            hipLaunchKernelGGL(k_half_sum, dim3(nb), dim3(256), 0, stream(), v.p, w.p, dout.p, (int64_t)n);
+        } else {
+            hipLaunchKernelGGL(k_half_sum, dim3(nb), dim3(256), 0, stream(), x.p, y.p, dout.p, (int64_t)n);
+            *final_it = iter;   // the DO variable after the loop (maxiter + 1 without an exit)
+        }
+        launch_check("twin_art");
+        dout.download(xout, n);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        if (consumed) *consumed = q;
     });
 }
 
@@ -683,9 +910,10 @@ extern "C" int fair_art(fair_csr *h, int32_t method, const double *b, int32_t nk
                 for (int ii = 0; ii < P.nord; ++ii) seq[P.nord + ii] = ord[P.nord - 1 - ii];
             if (steps_per_iter > 0) {
                 dseq.upload(seq.data(), steps_per_iter);
-                hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(), dseq.p,
-                                   (int64_t)steps_per_iter, h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p,
-                                   h->A.n, relax, box.lwrb, box.lb, box.upb, box.ub, dx.p);
+                hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
+                                   SweepChains{{dseq.p, nullptr}, {(int64_t)steps_per_iter, 0}, {dx.p, nullptr}},
+                                   h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p, h->A.n, relax, box.lwrb, box.lb,
+                                   box.upb, box.ub);
                 launch_check("art sweep");
             }
             if (rec < P.Kuse.size() && iter == P.Kuse[rec]) {   // :337-340

@@ -194,6 +194,79 @@ contains
         call release_dev(h, temp)
     end subroutine art
 
+    ! twin_art, AIRMs.f90:371-811 ('errorgauge' | 'mutualstep').  Randomized Kaczmarz draws
+    ! random_number values in the reference's order; the stopping rule decides how many are used,
+    ! so the generator state is saved, an upper bound of draws is taken, and afterwards the state
+    ! is restored and advanced by exactly the number consumed.
+    subroutine twin_art(xout, final_it, Xstore, Ystore, EG, art_method, stoprule, A, b, Kin, x0, y0, &
+                        order, slack, options)
+        character(*) :: art_method, stoprule
+        class(csr_matrix), intent(in) :: A
+        real(kind=8), intent(in) :: b(A%m)
+        integer, intent(in) :: Kin(:)
+        real(kind=8), intent(in), optional, target :: x0(A%n), y0(A%n)
+        integer, optional :: order(:), slack
+        class(opts), intent(in), optional :: options
+        real(kind=8), intent(out) :: xout(A%n)
+        integer :: final_it
+        real(kind=8), dimension(A%n, size(Kin)), intent(out) :: Xstore, Ystore
+        real(kind=8), intent(out) :: EG(maxval(Kin))
+        type(fair_opts_c) :: o
+        type(c_ptr) :: h, px0, py0, pord, pu
+        logical :: temp
+        integer(c_int32_t) :: meth, rule, nord, sl, fi
+        integer(c_int32_t), allocatable, target :: ordc(:)
+        integer(c_int32_t), allocatable :: kc(:)
+        integer(c_int64_t) :: cnt, used
+        real(c_double), allocatable, target :: u(:), xs(:), ys(:)
+        integer, allocatable :: state(:)
+        integer :: ns
+        meth = art_code(art_method)
+        select case (stoprule)
+        case ('errorgauge')
+            rule = 0
+        case ('mutualstep')
+            rule = 1
+        case default
+            stop "Unknown stopping rule. Use errorgauge or mutualstep."
+        end select
+        sl = 7
+        if (present(slack)) sl = slack
+        o = to_c(options)
+        call borrow_dev(A, h, temp)
+        pord = c_null_ptr
+        nord = 0
+        if (present(order)) then
+            ordc = order
+            nord = size(order)
+            pord = c_loc(ordc)
+        end if
+        kc = Kin
+        call fair_check(c_fair_twin_art_uniform_bound(h, meth, int(size(Kin), c_int32_t), kc, pord, nord, o, cnt))
+        call random_seed(size=ns)
+        allocate (state(ns), u(max(cnt, 1_c_int64_t)))
+        call random_seed(get=state)
+        if (cnt > 0) call random_number(u(1:cnt))
+        pu = c_null_ptr
+        if (cnt > 0) pu = c_loc(u)
+        px0 = c_null_ptr
+        py0 = c_null_ptr
+        if (present(x0)) then
+            xs = x0
+            px0 = c_loc(xs)
+        end if
+        if (present(y0)) then
+            ys = y0
+            py0 = c_loc(ys)
+        end if
+        call fair_check(c_fair_twin_art(h, meth, rule, b, int(size(Kin), c_int32_t), kc, px0, py0, pord, nord, sl, &
+                                        o, pu, cnt, xout, fi, Xstore, Ystore, EG, used))
+        final_it = fi
+        call random_seed(put=state)
+        if (used > 0) call random_number(u(1:used))
+        call release_dev(h, temp)
+    end subroutine twin_art
+
     ! sirt, AIRMs.f90:997-1127 ('cimmino' | 'sart'); SART leaves A's rows column-sorted.
     subroutine sirt(xout, sirt_method, A, b, Kin, x0, options)
         character(*) :: sirt_method

@@ -165,6 +165,17 @@ int fair_art(fair_csr *A, int32_t method, const double *b, int32_t nk, const int
 int fair_art_uniform_count(fair_csr *A, int32_t method, int32_t nk, const int32_t *Kin,
                            const int32_t *order, int32_t nord, const fair_opts *opts,
                            int64_t *count);
+/* twin_art, AIRMs.f90:371-811: stoprule 0 = 'errorgauge', 1 = 'mutualstep'; x0 / y0 / order
+ * may be NULL.  Xstore / Ystore: nk x n (row k = iterate at Kin[k]), EG: maxval(Kin) entries,
+ * *final_it as the reference returns it.  uniforms: the random_number draws in the reference's
+ * order (at most fair_twin_art_uniform_bound of them are read; *consumed, may be NULL, receives
+ * the number used). */
+int fair_twin_art(fair_csr *A, int32_t method, int32_t stoprule, const double *b, int32_t nk,
+                  const int32_t *Kin, const double *x0, const double *y0, const int32_t *order, int32_t nord,
+                  int32_t slack, const fair_opts *opts, const double *uniforms, int64_t nu, double *xout,
+                  int32_t *final_it, double *Xstore, double *Ystore, double *EG, int64_t *consumed);
+int fair_twin_art_uniform_bound(fair_csr *A, int32_t method, int32_t nk, const int32_t *Kin,
+                                const int32_t *order, int32_t nord, const fair_opts *opts, int64_t *count);
 /* basic_sweep's row steps, AIRMs.f90:840-856, over an explicit one-based row sequence, as one
  * persistent launch.  nrA: damped squared row norms (m).  d_x in HBM. */
 int fair_art_steps_dev(fair_csr *A, const int32_t *d_rows, int64_t nsteps, const double *d_b,
