@@ -52,6 +52,9 @@ SIGNATURES = {
     "fair_csr_download": [_P, _P, _P, _P],
     "fair_csr_device_arrays": [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)],
     "fair_csr_free": [_P],
+    "fair_matrix_store": [C.c_char_p, _I32, _I32, _I64, _P, _P, _P, _I32],
+    "fair_matrix_read_header": [C.c_char_p, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I64), C.POINTER(_I32)],
+    "fair_matrix_read": [C.c_char_p, _I64, _I32, _P, _P, _P],
     "fair_matvecmul": [_P, _P, _I32, _P, _I32],
     "fair_transvecmul": [_P, _P, _I32, _P, _I32],
     "fair_rowvecmult": [_P, _I32, _P, _I32, _P],

@@ -106,6 +106,77 @@ contains
     ! add_white_noise, system_generation.f90:709-745: per sample a uniform draw u, a Newton
     ! solve of Phi(x) = u from x = 0 with the reference's step f/df, df = sqrt(2/pi)exp(-x^2/2),
     ! then e = eta*||b||*e/||e|| and bt = b + e.
+    ! store_matrix, system_generation.f90:595-625: the records m, n, nnz, vA, JA, IA (Fortran
+    ! unformatted sequential), from the host components (device copies synchronised first).
+    subroutine store_matrix(A, inputfile, verbose)
+        character(*), intent(in) :: inputfile
+        class(sparse_matrix) :: A
+        logical, optional :: verbose
+        integer :: k
+        logical :: verb
+        verb = .false.
+        if (present(verbose)) verb = verbose
+        if (c_associated(A%dev)) call A%sync_host()
+        open (newunit=k, file=inputfile, form='unformatted')
+        if (verb) write (*, *) "Opening file: ", inputfile
+        write (k) A%m
+        write (k) A%n
+        write (k) A%nnz
+        write (k) A%vA
+        write (k) A%JA
+        write (k) A%IA
+        close (k)
+        if (verb) write (*, *) "done! Closing file."
+    end subroutine store_matrix
+
+    ! read_csr_matrix, system_generation.f90:627-665 (with a unit of its own: the reference opens
+    ! an unset unit number, :637).  Any device copy is dropped; the next product uploads.
+    subroutine read_csr_matrix(A, inputfile, verbose)
+        character(*), intent(in) :: inputfile
+        class(csr_matrix) :: A
+        logical, optional :: verbose
+        call read_dump(A, inputfile, .false., verbose)
+    end subroutine read_csr_matrix
+
+    ! read_csc_matrix, system_generation.f90:667-707
+    subroutine read_csc_matrix(A, inputfile, verbose)
+        character(*), intent(in) :: inputfile
+        class(csc_matrix) :: A
+        logical, optional :: verbose
+        call read_dump(A, inputfile, .true., verbose)
+    end subroutine read_csc_matrix
+
+    subroutine read_dump(A, inputfile, csc, verbose)
+        class(sparse_matrix) :: A
+        character(*), intent(in) :: inputfile
+        logical, intent(in) :: csc
+        logical, optional :: verbose
+        integer :: k
+        logical :: verb
+        verb = .false.
+        if (present(verbose)) verb = verbose
+        if (verb) write (*, *) "Opening file: ", inputfile
+        call A%free_device()
+        open (newunit=k, form="unformatted", file=inputfile, action="read")
+        read (k) A%m
+        read (k) A%n
+        read (k) A%nnz
+        if (allocated(A%vA)) deallocate (A%vA)
+        if (allocated(A%JA)) deallocate (A%JA)
+        if (allocated(A%IA)) deallocate (A%IA)
+        allocate (A%vA(A%nnz), A%JA(A%nnz))
+        if (csc) then
+            allocate (A%IA(0:A%n))
+        else
+            allocate (A%IA(0:A%m))
+        end if
+        read (k) A%vA
+        read (k) A%JA
+        read (k) A%IA
+        close (k)
+        if (verb) write (*, *) "Data collected. Closing file."
+    end subroutine read_dump
+
     subroutine add_white_noise(bt, e, b, eta)
         real(kind=8), intent(in) :: b(:), eta
         real(kind=8), dimension(size(b)) :: bt, e

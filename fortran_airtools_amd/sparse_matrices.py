@@ -9,10 +9,11 @@ callers of the reference read them directly (main.f90:288-291).
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
-from ._lib import FairError, FAIR_ERR_ARG, check, lib
+from ._lib import FairError, FAIR_ERR_ARG, check, lib, load
 
 
 def _ptr(a):
@@ -172,3 +173,45 @@ class CSRMatrix:
 # The reference's CSC tag type has no separate behaviour on the hot path.
 sparse_matrix = CSRMatrix
 CSR_matrix = CSRMatrix
+
+
+# ---- matrix dumps (store_matrix / read_csr_matrix, system_generation.f90:595-707) -------------
+def store_matrix_arrays(inputfile: str, m: int, n: int, IA, JA, vA) -> None:
+    """Write the reference's dump (Fortran unformatted records m, n, nnz, vA, JA, IA); host only."""
+    IA = np.ascontiguousarray(IA, np.int32)
+    JA = np.ascontiguousarray(JA, np.int32)
+    vA = np.ascontiguousarray(vA, np.float64)
+    if JA.shape != vA.shape:
+        raise FairError(FAIR_ERR_ARG, "JA and vA differ in length")
+    L = load()
+    check(L.fair_matrix_store(os.fsencode(inputfile), int(m), int(n), int(vA.shape[0]), _ptr(vA), _ptr(JA),
+                              _ptr(IA), int(IA.shape[0])))
+
+
+def read_matrix_arrays(inputfile: str):
+    """Read a dump written by store_matrix: returns (m, n, IA, JA, vA); host only."""
+    L = load()
+    m, n, nia = C.c_int32(), C.c_int32(), C.c_int32()
+    nnz = C.c_int64()
+    path = os.fsencode(inputfile)
+    check(L.fair_matrix_read_header(path, C.byref(m), C.byref(n), C.byref(nnz), C.byref(nia)))
+    IA = np.empty(nia.value, np.int32)
+    JA = np.empty(max(nnz.value, 1), np.int32)
+    vA = np.empty(max(nnz.value, 1), np.float64)
+    check(L.fair_matrix_read(path, nnz.value, nia.value, _ptr(vA), _ptr(JA), _ptr(IA)))
+    return m.value, n.value, IA, JA[:nnz.value], vA[:nnz.value]
+
+
+def store_matrix(A: CSRMatrix, inputfile: str) -> None:
+    """call store_matrix(A, inputfile), system_generation.f90:595-625."""
+    IA, JA, vA = A.components()
+    store_matrix_arrays(inputfile, A.m, A.n, IA, JA, vA)
+
+
+def read_csr_matrix(inputfile: str) -> CSRMatrix:
+    """call read_csr_matrix(A, inputfile), system_generation.f90:627-665, into device memory."""
+    m, n, IA, JA, vA = read_matrix_arrays(inputfile)
+    if IA.shape[0] != m + 1:
+        raise FairError(FAIR_ERR_ARG, "the dump is not a CSR matrix (IA has n+1 entries)")
+    return CSRMatrix.from_arrays(m, n, IA, JA, vA)
+

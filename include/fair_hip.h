@@ -75,6 +75,15 @@ int fair_csr_device_arrays(const fair_csr *A, const int32_t **IA, const int32_t 
                            const double **vA);
 int fair_csr_free(fair_csr *A);
 
+/* ---- matrix dumps (host only) --------------------------------------------------------------
+ * The Fortran unformatted records of store_matrix (system_generation.f90:595-625): m, n, nnz, vA,
+ * JA, IA (nia = m+1 for CSR, n+1 for CSC), readable by read_csr_matrix / read_csc_matrix
+ * (:627-707) and written by the reference readable here. */
+int fair_matrix_store(const char *path, int32_t m, int32_t n, int64_t nnz, const double *vA,
+                      const int32_t *JA, const int32_t *IA, int32_t nia);
+int fair_matrix_read_header(const char *path, int32_t *m, int32_t *n, int64_t *nnz, int32_t *nia);
+int fair_matrix_read(const char *path, int64_t nnz, int32_t nia, double *vA, int32_t *JA, int32_t *IA);
+
 /* ---- sparse_matrices: operations ---------------------------------------------------- */
 /* matvecmul -> sparsematmul/MMRS, sparse_matrices.f90:272-298, 349-368. */
 int fair_matvecmul(fair_csr *A, const double *v, int32_t nv, double *w, int32_t nw);

@@ -292,6 +292,7 @@ class Reference:
                               i, _F64P]
         L.ref_twin_art.argtypes = [i, i, i, i, _F64P, i, _I32P, i, i, i, _F64P, _F64P, i, i, i, d, d, d, d, i,
                                    _F64P, C.POINTER(i), _F64P, _F64P, _F64P, i]
+        L.ref_store.argtypes = [i, C.c_char_p, i]
         L.ref_rng_seed.argtypes = [i]
         L.ref_rng_draw.argtypes = [i, _F64P]
         L.ref_shepplogan.argtypes = [i, _F64P]
@@ -409,6 +410,10 @@ class Reference:
                        0.0 if ub is None else ub, 1.0 if lb is None else lb, relax, damp,
                        bool(rowprob), out)
         return out  # row k = iterate recorded at kin[k] (Fortran column k)
+
+    def store(self, slot, path):
+        p = os.fsencode(path)
+        self.L.ref_store(slot, p, len(p))
 
     def rng_seed(self, seed):
         self.L.ref_rng_seed(seed)

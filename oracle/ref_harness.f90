@@ -319,6 +319,18 @@ contains
         eg = egv(1:neg)
     end subroutine ref_twin_art
 
+    ! store_matrix (system_generation.f90:595-625) of a slot's matrix.
+    subroutine ref_store(slot, path, plen) bind(C, name='ref_store')
+        integer(c_int), value :: slot, plen
+        character(kind=c_char), intent(in) :: path(plen)
+        character(len=plen) :: fname
+        integer :: i
+        do i = 1, plen
+            fname(i:i) = path(i)
+        end do
+        call store_matrix(mats(slot), fname)
+    end subroutine ref_store
+
     ! Seed the runtime generator deterministically from one integer.
     subroutine ref_rng_seed(seed) bind(C, name='ref_rng_seed')
         integer(c_int), value :: seed

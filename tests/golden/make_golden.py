@@ -15,8 +15,10 @@ Outputs (numpy .npz, inputs and expected outputs only):
                      and Cimmino 200-iteration iterates
   twin_small.npz     (--twin) twin_art with both stopping rules on a small fan problem, with the
                      uniform streams drawn
+  dump_small.bin     (--dump) store_matrix's file of a small fan matrix (with dump_small.npz:
+                     the matrix it holds)
 
-usage: python tests/golden/make_golden.py [--c2 | --twin]
+usage: python tests/golden/make_golden.py [--c2 | --twin | --dump]
 """
 from __future__ import annotations
 
@@ -205,6 +207,14 @@ def write_twin(ref):
     np.savez_compressed(os.path.join(HERE, "twin_small.npz"), **d)
 
 
+def write_dump(ref):
+    g = cf.Geometry(n=12, p=17, nth=10, deg_start=0.37, deg_step=36.0)
+    IA, JA, vA = build(ref, 0, g.beam, g.n, g.p, g.nth, g.r, g.dw, g.sd, g.theta())
+    ref.store(0, os.path.join(HERE, "dump_small.bin"))
+    np.savez_compressed(os.path.join(HERE, "dump_small.npz"), m=np.array(g.m), n=np.array(g.ncols), IA=IA, JA=JA,
+                        vA=vA)
+
+
 def write_c2(ref):
     g = cf.C2
     IA, JA, vA, xt, b = problem(ref, g, cf.grains, 0.005, 2)
@@ -227,6 +237,9 @@ def main():
         return
     if "--twin" in sys.argv:
         write_twin(ref)
+        return
+    if "--dump" in sys.argv:
+        write_dump(ref)
         return
     write_small(ref)
     write_c1(ref)

@@ -84,3 +84,20 @@ def test_spmv_autotune_keeps_results():
     after = (A.matvecmul(x), A.transvecmul(y), airms.sirt("sart", A, b, [5], options=o))
     for u, v in zip(before, after):
         assert u.tobytes() == v.tobytes()
+
+
+def test_store_and_read_device_matrix(tmp_path):
+    """A device-built matrix stored with store_matrix and read back with read_csr_matrix gives
+    the same products (rows as the caller sees them: sorted after SART, as in the reference)."""
+    from fortran_airtools_amd import airms, configs as cf, sparse_matrices as sm, system_generation as sg
+    g = cf.C1
+    A = sg.fanlineartomo(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta()))
+    airms.sirt("sart", A, A.matvecmul(cf.shepplogan(g.n)), [1])     # sorts the caller's rows
+    path = str(tmp_path / "A.bin")
+    sm.store_matrix(A, path)
+    B = sm.read_csr_matrix(path)
+    IA, JA, vA = A.components()
+    IB, JB, vB = B.components()
+    assert IA.tobytes() == IB.tobytes() and JA.tobytes() == JB.tobytes() and vA.tobytes() == vB.tobytes()
+    x = np.random.default_rng(5).random(A.n)
+    assert A.matvecmul(x).tobytes() == B.matvecmul(x).tobytes()
