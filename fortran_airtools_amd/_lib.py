@@ -55,6 +55,8 @@ SIGNATURES = {
     "fair_matrix_store": [C.c_char_p, _I32, _I32, _I64, _P, _P, _P, _I32],
     "fair_matrix_read_header": [C.c_char_p, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I64), C.POINTER(_I32)],
     "fair_matrix_read": [C.c_char_p, _I64, _I32, _P, _P, _P],
+    "fair_absorbance": [_P, _I64, _I64, _I64, _P],
+    "fair_absorbance_dev": [_P, _I64, _I64, _I64, _P],
     "fair_matvecmul": [_P, _P, _I32, _P, _I32],
     "fair_transvecmul": [_P, _P, _I32, _P, _I32],
     "fair_rowvecmult": [_P, _I32, _P, _I32, _P],

@@ -1,8 +1,9 @@
 ! fair_extensions.f90 -- Fortran access to the additions that stand beside the drop-in path:
 ! the FP32 precision variant of sirt, sirt over many right-hand sides at once (batched, with
-! an autotuner), and the matrix-free fan-beam projector.  None has a counterpart in the
-! reference's modules; they answer its to-dos "blocking support" and "make matrix-free
-! implementations" (README.md:6-7) and keep the reference's argument conventions.
+! an autotuner), the matrix-free fan-beam projector, and bt = -log(T) of measured data on the
+! device.  None has a counterpart in the reference's modules; the first three answer its to-dos
+! "blocking support" and "make matrix-free implementations" (README.md:6-7), and all keep the
+! reference's argument conventions.
 module fair_extensions
     use iso_c_binding
     use fair_capi
@@ -11,7 +12,7 @@ module fair_extensions
     use AIRMs
     implicit none
     private
-    public :: sirt_f32, sirt_batch, autotune_batch, fan_projector
+    public :: sirt_f32, sirt_batch, autotune_batch, fan_projector, absorbance
 
     ! A of fanlineartomo(dat), applied without storing it (FP32 values, FP64 ray geometry)
     type fan_projector
@@ -28,6 +29,13 @@ module fair_extensions
     end type fan_projector
 
     interface
+        function c_fair_absorbance(T, p, nth, skip, b) bind(C, name='fair_absorbance') result(s)
+            import :: c_int, c_int64_t, c_double
+            real(c_double), intent(in) :: T(*)
+            integer(c_int64_t), value :: p, nth, skip
+            real(c_double) :: b(*)
+            integer(c_int) :: s
+        end function
         function c_fair_sirt_f32(A, method, b, maxits, x0, o, xout) bind(C, name='fair_sirt_f32') result(s)
             import :: c_ptr, c_int, c_int32_t, c_float, fair_opts_c
             type(c_ptr), value :: A, x0
@@ -281,4 +289,19 @@ contains
         P%h = c_null_ptr
     end subroutine mf_free
 
+
+    ! bt = -log(T) of main.f90:270 on the device, after the angle cull of main.f90:229-245 when
+    ! skip > 0 (T holds p values per angle for nth angles).
+    function absorbance(T, p, nth, skip) result(b)
+        real(kind=8), intent(in) :: T(:)
+        integer, intent(in) :: p, nth, skip
+        real(kind=8), allocatable :: b(:)
+        integer :: keep
+        if (size(T) /= p*nth) stop "Data has the wrong size, or p and nth are incorrect."
+        keep = nth
+        if (skip > 0 .and. nth > 0) keep = (nth - 1)/skip + 1
+        allocate (b(keep*p))
+        call fair_ensure_init()
+        call fair_check(c_fair_absorbance(T, int(p, c_int64_t), int(nth, c_int64_t), int(skip, c_int64_t), b))
+    end function absorbance
 end module fair_extensions

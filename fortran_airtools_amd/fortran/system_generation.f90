@@ -71,7 +71,7 @@ contains
     ! then "label count" + angles in degrees, then "label count" + transmission data.
     subroutine read_data(dat, inputfile, verbose)
         class(data), intent(out) :: dat
-        character(30), intent(in) :: inputfile
+        character(*), intent(in) :: inputfile
         logical, optional :: verbose
         character(30) :: label
         integer :: cnt, i, u
@@ -103,9 +103,47 @@ contains
         if (verb) write (*, *) "Closing file: ", trim(inputfile)
     end subroutine read_data
 
-    ! add_white_noise, system_generation.f90:709-745: per sample a uniform draw u, a Newton
-    ! solve of Phi(x) = u from x = 0 with the reference's step f/df, df = sqrt(2/pi)exp(-x^2/2),
-    ! then e = eta*||b||*e/||e|| and bt = b + e.
+    ! read_ref, system_generation.f90:541-565: n records of one real each (a reference image).
+    subroutine read_ref(xref, inputfile, n, verbose)
+        integer :: n
+        real(kind=8) :: xref(n)
+        character(*), intent(in) :: inputfile
+        logical, optional :: verbose
+        integer :: i, u
+        logical :: verb
+        verb = .false.
+        if (present(verbose)) verb = verbose
+        if (verb) write (*, *) "Opening file: ", trim(inputfile)
+        open (newunit=u, file=trim(inputfile), action='read')
+        do i = 1, n
+            read (u, *) xref(i)
+        end do
+        close (u)
+        if (verb) write (*, *) "Closing file: ", trim(inputfile)
+    end subroutine read_ref
+
+    ! read_ordering, system_generation.f90:567-593: a "label l" record, then l records of one
+    ! integer each (a row order for art).
+    subroutine read_ordering(ordering, inputfile, verbose)
+        character(*), intent(in) :: inputfile
+        integer, allocatable :: ordering(:)
+        logical, optional :: verbose
+        character(30) :: label
+        integer :: cnt, i, u
+        logical :: verb
+        verb = .false.
+        if (present(verbose)) verb = verbose
+        if (verb) write (*, *) "Opening file: ", trim(inputfile)
+        open (newunit=u, file=trim(inputfile), action='read')
+        read (u, *) label, cnt
+        if (allocated(ordering)) deallocate (ordering)
+        allocate (ordering(cnt))
+        do i = 1, cnt
+            read (u, *) ordering(i)
+        end do
+        close (u)
+    end subroutine read_ordering
+
     ! store_matrix, system_generation.f90:595-625: the records m, n, nnz, vA, JA, IA (Fortran
     ! unformatted sequential), from the host components (device copies synchronised first).
     subroutine store_matrix(A, inputfile, verbose)
@@ -177,6 +215,9 @@ contains
         if (verb) write (*, *) "Data collected. Closing file."
     end subroutine read_dump
 
+    ! add_white_noise, system_generation.f90:709-745: per sample a uniform draw u, a Newton
+    ! solve of Phi(x) = u from x = 0 with the reference's step f/df, df = sqrt(2/pi)exp(-x^2/2),
+    ! then e = eta*||b||*e/||e|| and bt = b + e.
     subroutine add_white_noise(bt, e, b, eta)
         real(kind=8), intent(in) :: b(:), eta
         real(kind=8), dimension(size(b)) :: bt, e

@@ -84,6 +84,13 @@ int fair_matrix_store(const char *path, int32_t m, int32_t n, int64_t nnz, const
 int fair_matrix_read_header(const char *path, int32_t *m, int32_t *n, int64_t *nnz, int32_t *nia);
 int fair_matrix_read(const char *path, int64_t nnz, int32_t nia, double *vA, int32_t *JA, int32_t *IA);
 
+/* ---- measured data -------------------------------------------------------------------------
+ * The file mode of main.f90: the angle cull of main.f90:229-245 (keep angles 1, 1+skip, ... of
+ * the (p, nth) transmission data T; skip = 0 keeps all) and b = -log(T) (main.f90:270), on the
+ * device.  b holds ceil(nth/skip)*p values (nth*p for skip = 0). */
+int fair_absorbance(const double *T, int64_t p, int64_t nth, int64_t skip, double *b);
+int fair_absorbance_dev(const double *T, int64_t p, int64_t nth, int64_t skip, double *b);
+
 /* ---- sparse_matrices: operations ---------------------------------------------------- */
 /* matvecmul -> sparsematmul/MMRS, sparse_matrices.f90:272-298, 349-368. */
 int fair_matvecmul(fair_csr *A, const double *v, int32_t nv, double *w, int32_t nw);

@@ -299,6 +299,12 @@ class Reference:
         L.ref_linspace.argtypes = [d, d, i, _F64P]
         L.ref_sum.restype = d
         L.ref_sum.argtypes = [i, _F64P]
+        L.ref_read_data.argtypes = [C.c_char_p, i, _I32P, _F64P]
+        L.ref_data_arrays.argtypes = [i, _F64P, i, _F64P]
+        L.ref_file_mode.argtypes = [i, C.c_char_p, i, i, _I32P, _F64P]
+        L.ref_file_mode_b.argtypes = [i, _F64P]
+        L.ref_read_ref.argtypes = [C.c_char_p, i, i, _F64P]
+        L.ref_read_ordering.argtypes = [C.c_char_p, i, i, C.POINTER(i), _I32P]
 
     def build(self, slot, kind, n, p, nth, r, dw, sd, theta):
         theta = np.ascontiguousarray(theta, np.float64)
@@ -435,3 +441,47 @@ class Reference:
 
     def fsum(self, v):
         return self.L.ref_sum(len(v), np.ascontiguousarray(v, np.float64))
+
+    @staticmethod
+    def _name30(path):
+        p = os.fsencode(path)
+        if len(p) > 30:
+            raise ValueError("the reference's file-name arguments are character(30)")
+        return p
+
+    def _data_arrays(self, nth, nt):
+        theta, T = np.empty(nth), np.empty(nt)
+        self.L.ref_data_arrays(nth, theta, nt, T)
+        return theta, T
+
+    def read_data(self, path):
+        """read_data: ((n, p, nth), (r, dw, sd), theta, T)."""
+        p = self._name30(path)
+        ig, rg = np.zeros(3, np.int32), np.zeros(3)
+        self.L.ref_read_data(p, len(p), ig, rg)
+        n, pp, nth = (int(v) for v in ig)
+        return (n, pp, nth), tuple(float(v) for v in rg), *self._data_arrays(nth, nth * pp)
+
+    def file_mode(self, slot, path, skip=0):
+        """main.f90's file mode: ((n, p, nth), (r, dw, sd, grid_size), theta, T, bt); A in slot."""
+        p = self._name30(path)
+        ig, rg = np.zeros(3, np.int32), np.zeros(4)
+        self.L.ref_file_mode(slot, p, len(p), skip, ig, rg)
+        n, pp, nth = (int(v) for v in ig)
+        theta, T = self._data_arrays(nth, nth * pp)
+        bt = np.empty(nth * pp)
+        self.L.ref_file_mode_b(nth * pp, bt)
+        return (n, pp, nth), tuple(float(v) for v in rg), theta, T, bt
+
+    def read_ref(self, path, n):
+        p = self._name30(path)
+        x = np.empty(n)
+        self.L.ref_read_ref(p, len(p), n, x)
+        return x
+
+    def read_ordering(self, path, cap):
+        p = self._name30(path)
+        l = C.c_int()
+        out = np.zeros(max(cap, 1), np.int32)
+        self.L.ref_read_ordering(p, len(p), cap, C.byref(l), out)
+        return out[: min(l.value, cap)], l.value

@@ -16,7 +16,10 @@ module ref_harness
     use phantomgallery
     implicit none
     integer, parameter :: NSLOT = 8
-    type(csr_matrix), save :: mats(NSLOT)
+    type(csr_matrix), save :: mats(0:NSLOT-1)   ! slots 0 .. NSLOT-1, as the Python side numbers them
+    ! data read by ref_read_data / ref_file_mode, and the file mode's right-hand side
+    type(data), save :: fm_dat
+    real(c_double), allocatable, save :: fm_b(:)
 contains
 
     subroutine make_geom(dat, n, p, nth, r, dw, sd, theta)
@@ -378,4 +381,96 @@ contains
         real(c_double) :: s
         s = sum(v)
     end function ref_sum
+    ! A path as the reference's character(30) file-name arguments take it (blank padded).
+    function name30(path, plen) result(fname)
+        integer(c_int), intent(in) :: plen
+        character(kind=c_char), intent(in) :: path(plen)
+        character(30) :: fname
+        integer :: i
+        fname = ' '
+        do i = 1, min(plen, 30)
+            fname(i:i) = path(i)
+        end do
+    end function name30
+
+    ! read_data (system_generation.f90:494-539) into fm_dat; ig = n, p, nth; rg = r, dw, sd.
+    subroutine ref_read_data(path, plen, ig, rg) bind(C, name='ref_read_data')
+        integer(c_int), value :: plen
+        character(kind=c_char), intent(in) :: path(plen)
+        integer(c_int), intent(out) :: ig(3)
+        real(c_double), intent(out) :: rg(3)
+        call read_data(fm_dat, name30(path, plen))
+        ig = [fm_dat%n, fm_dat%p, fm_dat%nth]
+        rg = [fm_dat%r, fm_dat%dw, fm_dat%sd]
+    end subroutine ref_read_data
+
+    ! theta (nth) and T (nt) of fm_dat.
+    subroutine ref_data_arrays(nth, theta, nt, T) bind(C, name='ref_data_arrays')
+        integer(c_int), value :: nth, nt
+        real(c_double), intent(out) :: theta(nth), T(nt)
+        theta = fm_dat%theta
+        T = fm_dat%T
+    end subroutine ref_data_arrays
+
+    ! The file mode of main.f90:222-270 driven through the reference's own procedures: read_data,
+    ! the angle cull for skip > 0, the grid_size rescale, A = fanlineartomo(dat) in the slot,
+    ! scaled by multiply_by_scalar(grid_size/N), and bt = -log(T).  ig = n, p, nth and
+    ! rg = r, dw, sd, grid_size after these steps; ref_data_arrays and ref_file_mode_b read the
+    ! culled data and bt.
+    subroutine ref_file_mode(slot, path, plen, skip, ig, rg) bind(C, name='ref_file_mode')
+        integer(c_int), value :: slot, plen, skip
+        character(kind=c_char), intent(in) :: path(plen)
+        integer(c_int), intent(out) :: ig(3)
+        real(c_double), intent(out) :: rg(4)
+        real(kind=8), allocatable :: grid(:, :), keep(:)
+        real(kind=8) :: grid_size
+        integer :: cnt
+        call read_data(fm_dat, name30(path, plen))
+        if (skip > 0) then
+            grid = reshape(fm_dat%T, [fm_dat%p, fm_dat%nth])
+            cnt = size(fm_dat%theta(1:fm_dat%nth:skip))
+            keep = fm_dat%theta(1:fm_dat%nth:skip)
+            call move_alloc(keep, fm_dat%theta)
+            fm_dat%T = reshape(grid(:, 1:fm_dat%nth:skip), [cnt*fm_dat%p])
+            fm_dat%nth = cnt
+        end if
+        grid_size = fm_dat%dw*fm_dat%r/fm_dat%sd
+        fm_dat%r = fm_dat%r/grid_size
+        fm_dat%dw = fm_dat%dw/grid_size
+        fm_dat%sd = fm_dat%sd/grid_size
+        mats(slot) = fanlineartomo(fm_dat)
+        call mats(slot)%multiply_by_scalar(grid_size/fm_dat%n)
+        fm_b = -log(fm_dat%T)
+        ig = [fm_dat%n, fm_dat%p, fm_dat%nth]
+        rg = [fm_dat%r, fm_dat%dw, fm_dat%sd, grid_size]
+    end subroutine ref_file_mode
+
+    subroutine ref_file_mode_b(m, b) bind(C, name='ref_file_mode_b')
+        integer(c_int), value :: m
+        real(c_double), intent(out) :: b(m)
+        b = fm_b
+    end subroutine ref_file_mode_b
+
+    ! read_ref (system_generation.f90:541-565).
+    subroutine ref_read_ref(path, plen, n, x) bind(C, name='ref_read_ref')
+        integer(c_int), value :: plen, n
+        character(kind=c_char), intent(in) :: path(plen)
+        real(c_double), intent(out) :: x(n)
+        real(kind=8), allocatable :: xr(:)
+        allocate (xr(n))
+        call read_ref(xr, name30(path, plen), n)
+        x = xr
+    end subroutine ref_read_ref
+
+    ! read_ordering (system_generation.f90:567-593): l entries, at most cap copied to order.
+    subroutine ref_read_ordering(path, plen, cap, l, order) bind(C, name='ref_read_ordering')
+        integer(c_int), value :: plen, cap
+        character(kind=c_char), intent(in) :: path(plen)
+        integer(c_int), intent(out) :: l
+        integer(c_int), intent(out) :: order(cap)
+        integer, allocatable :: ordering(:)
+        call read_ordering(ordering, name30(path, plen))
+        l = size(ordering)
+        order(1:min(l, cap)) = ordering(1:min(l, cap))
+    end subroutine ref_read_ordering
 end module ref_harness

@@ -17,8 +17,15 @@ Outputs (numpy .npz, inputs and expected outputs only):
                      uniform streams drawn
   dump_small.bin     (--dump) store_matrix's file of a small fan matrix (with dump_small.npz:
                      the matrix it holds)
+  fan_data_small.txt (--data) a measured-data file in read_data's text format (physical units,
+                     transmission values of a phantom, mixed number spellings), with
+  fan_data_small.npz what the reference's read_data returns for it and the file mode of main.f90
+                     (cull, grid_size rescale, scaled fanlineartomo, bt = -log T) for skip 0 and
+                     4, with 20 SART iterations on each
+  ref_image_small.txt, order_small.txt (--data) read_ref / read_ordering inputs, their results
+                     in fan_data_small.npz
 
-usage: python tests/golden/make_golden.py [--c2 | --twin | --dump]
+usage: python tests/golden/make_golden.py [--c2 | --twin | --dump | --data]
 """
 from __future__ import annotations
 
@@ -215,6 +222,75 @@ def write_dump(ref):
                         vA=vA)
 
 
+def spell_real(v: float, k: int) -> str:
+    """Number spellings list-directed input accepts, rotated by k."""
+    form = k % 5
+    if form == 0:
+        return f"{v:.17e}"
+    if form == 1:
+        return f"{v:.17g}"
+    if form == 2:
+        return f"{v:.16E}".replace("E", "D")
+    if form == 3:
+        return f"  {v:.17e},"
+    return f"{v:.17e}  trailing words"
+
+
+def write_data(ref):
+    # A fan-beam scan in millimetres: source 500 mm from the centre, a 400 mm detector of p
+    # pixels 1000 mm from the source; 60 views every 6 degrees.
+    n, p, nth, R, dw, sd = 32, 49, 60, 500.0, 400.0, 1000.0
+    deg = np.arange(nth) * 6.0
+    grid = dw * R / sd
+    ref.build(0, 0, n, p, nth, R / grid, dw / grid, sd / grid, deg / 180.0 * PI)
+    ref.scale(0, grid / n)
+    mu = 0.02                                        # attenuation per mm inside the phantom
+    line = ref.matvec(0, mu * ref.shepplogan(n))
+    rng = np.random.default_rng(17)
+    T = np.exp(-line) * (1.0 + 0.002 * rng.standard_normal(line.shape[0]))
+    lines = ["n 32", "p: 49", "'nth' 60", "R 5.0d2", "dw, 4.0E+02", "sd  1000.", "angles 60"]
+    for k, a in enumerate(deg):
+        lines.append(["0.0", "6", "1.2d1"][k] if k < 3 else spell_real(float(a), k))
+        if k == 10:
+            lines.append("")
+    lines.append(f"transmission {nth * p}")
+    lines += [spell_real(float(t), k) for k, t in enumerate(T)]
+    with open(os.path.join(HERE, "fan_data_small.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+    ximg = ref.shepplogan(16)
+    with open(os.path.join(HERE, "ref_image_small.txt"), "w") as f:
+        f.write("\n".join(spell_real(float(v), k) for k, v in enumerate(ximg)) + "\n")
+    order = np.random.default_rng(3).permutation(40)[:30] + 1
+    with open(os.path.join(HERE, "order_small.txt"), "w") as f:
+        f.write("ordering 30\n" + "\n".join((" +" if k % 3 == 0 else " ") + str(v) for k, v in enumerate(order)) + "\n")
+
+    # The reference reads each file; paths are character(30) there, so go through a short name.
+    tmp = "/tmp/fair_golden_data.txt"
+    d = {}
+    for name, src in (("data", "fan_data_small.txt"), ("img", "ref_image_small.txt"), ("ord", "order_small.txt")):
+        with open(os.path.join(HERE, src), "rb") as fi, open(tmp, "wb") as fo:
+            fo.write(fi.read())
+        if name == "data":
+            geom, rdwsd, theta, Tr = ref.read_data(tmp)
+            d.update({"read/geom": np.array(geom), "read/rdwsd": np.array(rdwsd), "read/theta": theta, "read/T": Tr})
+            for skip in (0, 4):
+                geom, rg, theta, Tc, bt = ref.file_mode(0, tmp, skip)
+                IA, JA, vA = ref.get(0)
+                pre = f"fm{skip}/"
+                d.update({pre + "geom": np.array(geom), pre + "rg": np.array(rg), pre + "theta": theta,
+                          pre + "T": Tc, pre + "bt": bt, pre + "IA": IA})
+                d.update(matrix_summary(pre, IA, JA, vA, 37))
+                d[pre + "sart_20"] = ref.sirt(0, "sart", bt, 20, lb=0.0, relax=1.0)
+        elif name == "img":
+            d["img/x"] = ref.read_ref(tmp, 256)
+        else:
+            d["ord/order"], l = ref.read_ordering(tmp, 64)
+            assert l == 30
+    os.remove(tmp)
+    np.savez_compressed(os.path.join(HERE, "fan_data_small.npz"), **d)
+
+
 def write_c2(ref):
     g = cf.C2
     IA, JA, vA, xt, b = problem(ref, g, cf.grains, 0.005, 2)
@@ -240,6 +316,9 @@ def main():
         return
     if "--dump" in sys.argv:
         write_dump(ref)
+        return
+    if "--data" in sys.argv:
+        write_data(ref)
         return
     write_small(ref)
     write_c1(ref)
