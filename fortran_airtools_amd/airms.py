@@ -170,11 +170,14 @@ def art_uniform_count(art_method: str, A: CSRMatrix, Kin, order=None, options: O
 
 
 def art(art_method: str, A: CSRMatrix, b, Kin, x0=None, order=None, options: Opts | None = None,
-        uniforms=None, seed: int = 0) -> np.ndarray:
-    """call art(Xout, art_method, A, b, Kin, x0, order=order, options=options), AIRMs.f90:59-369
-    (without the xref early stop).  Returns Xout as an array of shape (size(Kin), n): row k is the
-    iterate recorded at Kin(k).  `uniforms` are the RANDOM_NUMBER draws of randomized Kaczmarz in
-    the reference's order; seeded uniform draws are used when not supplied."""
+        uniforms=None, seed: int = 0, xref=None, info: dict | None = None) -> np.ndarray:
+    """call art(Xout, art_method, A, b, Kin, x0, xref, order, options), AIRMs.f90:59-369.
+    Returns Xout as an array of shape (size(Kin), n): row k is the iterate recorded at Kin(k).
+    With xref the run stops once ||x - xref|| grows over an iteration and the last row holds the
+    iterate before that iteration (:322, 330-333); rows not reached are zero.  `uniforms` are the
+    RANDOM_NUMBER draws of randomized Kaczmarz in the reference's order; seeded uniform draws are
+    used when not supplied.  `info`, when given, receives 'exit_iter' (0: no early stop) and
+    'consumed' (draws used) of an xref run."""
     meth, K, ordv = _art_args(art_method, Kin, order)
     b = _f64(b, A.m)
     x0a = None if x0 is None else _f64(x0, A.n)
@@ -183,9 +186,18 @@ def art(art_method: str, A: CSRMatrix, b, Kin, x0=None, order=None, options: Opt
         uniforms = np.random.default_rng(seed).random(art_uniform_count(art_method, A, Kin, order, options))
     u = None if uniforms is None else np.ascontiguousarray(uniforms, np.float64)
     out = np.zeros((K.shape[0], A.n))
-    check(lib().fair_art(A.handle, meth, _ptr(b), K.shape[0], _ptr(K), _ptr(x0a), _ptr(ordv),
-                         0 if ordv is None else ordv.shape[0], C.byref(o), _ptr(u),
-                         0 if u is None else u.shape[0], _ptr(out)))
+    nord = 0 if ordv is None else ordv.shape[0]
+    if xref is None:
+        check(lib().fair_art(A.handle, meth, _ptr(b), K.shape[0], _ptr(K), _ptr(x0a), _ptr(ordv), nord,
+                             C.byref(o), _ptr(u), 0 if u is None else u.shape[0], _ptr(out)))
+        return out
+    xr = _f64(xref, A.n)
+    ex, used = C.c_int32(), C.c_int64()
+    check(lib().fair_art_oracle(A.handle, meth, _ptr(b), K.shape[0], _ptr(K), _ptr(x0a), _ptr(xr), _ptr(ordv),
+                                nord, C.byref(o), _ptr(u), 0 if u is None else u.shape[0], _ptr(out),
+                                C.byref(ex), C.byref(used)))
+    if info is not None:
+        info["exit_iter"], info["consumed"] = ex.value, used.value
     return out
 
 

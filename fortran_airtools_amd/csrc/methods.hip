@@ -862,65 +862,117 @@ extern "C" int fair_twin_art(fair_csr *h, int32_t method, int32_t stoprule, cons
     });
 }
 
+namespace {
+
+// art, AIRMs.f90:59-369.  With xref (the reference's oracle), the iterate before each iteration is
+// kept and the run stops once the distance to xref grows: Xout(:, size(Kin)) = that iterate
+// (:322, 330-333).  norm2(xp - xo) of one iteration is norm2(x - xo) of the one before, so one
+// device reduction per iteration suffices.  Rows of Xout that are not reached stay zero.
+void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int32_t *Kin, const double *x0,
+             const double *xref, const int32_t *order, int32_t nord, const fair_opts *opts, const double *uniforms,
+             int64_t nu, double *Xout, int32_t *exit_iter, int64_t *consumed) {
+    check_handle(h);
+    FAIR_CHECK(b && Xout, FAIR_ERR_ARG, "null argument");
+    DBuf<double> d_nrA;
+    ArtPlan P = plan_art(h, method, nk, Kin, order, nord, opts, d_nrA);
+    // Without xref every draw of the run is needed up front; with xref the run may stop early,
+    // so the draws are checked as each iteration takes them.
+    const int64_t need = uniforms_needed(P);
+    const int64_t per_iter = P.isrand ? 2 * (int64_t)P.nord + (P.rowprob ? P.nord : 0) : 0;
+    FAIR_CHECK((xref ? std::min(need, 2 * (int64_t)P.nord + per_iter) : need) <= nu && (need == 0 || uniforms),
+               FAIR_ERR_ARG, "too few uniforms: need " + std::to_string(need));
+    const int32_t m = h->A.m, n = h->A.n;
+    const Box box = make_box(opts);
+    const double relax = opts ? opts->relaxpar : 1.0;
+    for (size_t k = 0; k < (size_t)nk * (size_t)n; ++k) Xout[k] = 0.0;
+    DBuf<double> db(m > 0 ? m : 1), dx(n > 0 ? n : 1);
+    db.upload(b, m);
+    if (x0) dx.upload(x0, n);
+    else fill(dx.p, n, 0.0);
+    DBuf<double> dxr, dxp, part, out;
+    double prev = 0.0;
+    if (xref) {
+        dxr.alloc(n > 0 ? n : 1);
+        dxp.alloc(n > 0 ? n : 1);
+        part.alloc(RED_BLOCKS);
+        out.alloc(1);
+        dxr.upload(xref, n);
+        prev = std::sqrt(dev_sum(n, SqDiff{dx.p, dxr.p}, part, out));
+    }
+    const int steps_per_iter = P.method == FAIR_SYMKACZMARZ ? 2 * P.nord : P.nord;
+    std::vector<int32_t> seq(std::max(1, steps_per_iter));
+    DBuf<int32_t> dseq(seq.size());
+    std::vector<int32_t> ord = P.ord;
+    int64_t q = 0;
+    if (P.isrand) {                                   // :297
+        scramble(P.nord, uniforms + q, ord.data());
+        q += 2 * (int64_t)P.nord;
+    }
+    size_t rec = 0;
+    if (exit_iter) *exit_iter = 0;
+    const unsigned nb = grid_for(n, 256);
+    for (int iter = 1; iter <= P.maxiter; ++iter) {
+        FAIR_CHECK(q + per_iter <= nu || !P.isrand, FAIR_ERR_ARG, "too few uniforms: need " + std::to_string(need));
+        if (P.isrand) {                               // :320
+            scramble(P.nord, uniforms + q, ord.data());
+            q += 2 * (int64_t)P.nord;
+        }
+        for (int ii = 0; ii < P.nord; ++ii) {
+            if (P.rowprob) {                          // :828-835 (first ri with cumul(ri) >= u)
+                double u = uniforms[q++];
+                auto it = std::lower_bound(P.cumul.begin(), P.cumul.end(), u);
+                int ri = (int)(it - P.cumul.begin()) + 1;
+                seq[ii] = std::min(ri, (int)m);
+            } else {
+                seq[ii] = ord[ii];
+            }
+        }
+        if (P.method == FAIR_SYMKACZMARZ)             // :326-328
+            for (int ii = 0; ii < P.nord; ++ii) seq[P.nord + ii] = ord[P.nord - 1 - ii];
+        if (xref && n > 0) hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, stream(), dx.p, dxp.p, (int64_t)n);
+        if (steps_per_iter > 0) {
+            dseq.upload(seq.data(), steps_per_iter);
+            hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
+                               SweepChains{{dseq.p, nullptr}, {(int64_t)steps_per_iter, 0}, {dx.p, nullptr}},
+                               h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p, h->A.n, relax, box.lwrb, box.lb,
+                               box.upb, box.ub);
+            launch_check("art sweep");
+        }
+        if (xref) {                                   // :330-333
+            const double cur = std::sqrt(dev_sum(n, SqDiff{dx.p, dxr.p}, part, out));
+            if (cur > prev) {
+                dxp.download(Xout + (size_t)(nk - 1) * n, n);
+                FAIR_HIP(hipStreamSynchronize(stream()));
+                if (exit_iter) *exit_iter = iter;
+                break;
+            }
+            prev = cur;
+        }
+        if (rec < P.Kuse.size() && iter == P.Kuse[rec]) {   // :337-340
+            dx.download(Xout + rec * (size_t)n, n);
+            ++rec;
+        }
+        FAIR_HIP(hipStreamSynchronize(stream()));
+    }
+    if (consumed) *consumed = q;
+}
+
+}  // namespace
+
 extern "C" int fair_art(fair_csr *h, int32_t method, const double *b, int32_t nk, const int32_t *Kin,
                         const double *x0, const int32_t *order, int32_t nord, const fair_opts *opts,
                         const double *uniforms, int64_t nu, double *Xout) {
     return guard([&] {
-        check_handle(h);
-        FAIR_CHECK(b && Xout, FAIR_ERR_ARG, "null argument");
-        DBuf<double> d_nrA;
-        ArtPlan P = plan_art(h, method, nk, Kin, order, nord, opts, d_nrA);
-        const int64_t need = uniforms_needed(P);
-        FAIR_CHECK(need <= nu && (need == 0 || uniforms), FAIR_ERR_ARG,
-                   "too few uniforms: need " + std::to_string(need));
-        const int32_t m = h->A.m, n = h->A.n;
-        const Box box = make_box(opts);
-        const double relax = opts ? opts->relaxpar : 1.0;
-        DBuf<double> db(m > 0 ? m : 1), dx(n > 0 ? n : 1);
-        db.upload(b, m);
-        if (x0) dx.upload(x0, n);
-        else fill(dx.p, n, 0.0);
-        const int steps_per_iter = P.method == FAIR_SYMKACZMARZ ? 2 * P.nord : P.nord;
-        std::vector<int32_t> seq(std::max(1, steps_per_iter));
-        DBuf<int32_t> dseq(seq.size());
-        std::vector<int32_t> ord = P.ord;
-        int64_t q = 0;
-        if (P.isrand) {                                   // :297
-            scramble(P.nord, uniforms + q, ord.data());
-            q += 2 * (int64_t)P.nord;
-        }
-        std::vector<double> snap(n);
-        size_t rec = 0;
-        for (int iter = 1; iter <= P.maxiter; ++iter) {
-            if (P.isrand) {                               // :320
-                scramble(P.nord, uniforms + q, ord.data());
-                q += 2 * (int64_t)P.nord;
-            }
-            for (int ii = 0; ii < P.nord; ++ii) {
-                if (P.rowprob) {                          // :828-835 (first ri with cumul(ri) >= u)
-                    double u = uniforms[q++];
-                    auto it = std::lower_bound(P.cumul.begin(), P.cumul.end(), u);
-                    int ri = (int)(it - P.cumul.begin()) + 1;
-                    seq[ii] = std::min(ri, (int)m);
-                } else {
-                    seq[ii] = ord[ii];
-                }
-            }
-            if (P.method == FAIR_SYMKACZMARZ)             // :326-328
-                for (int ii = 0; ii < P.nord; ++ii) seq[P.nord + ii] = ord[P.nord - 1 - ii];
-            if (steps_per_iter > 0) {
-                dseq.upload(seq.data(), steps_per_iter);
-                hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
-                                   SweepChains{{dseq.p, nullptr}, {(int64_t)steps_per_iter, 0}, {dx.p, nullptr}},
-                                   h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p, h->A.n, relax, box.lwrb, box.lb,
-                                   box.upb, box.ub);
-                launch_check("art sweep");
-            }
-            if (rec < P.Kuse.size() && iter == P.Kuse[rec]) {   // :337-340
-                dx.download(Xout + rec * (size_t)n, n);
-                ++rec;
-            }
-            FAIR_HIP(hipStreamSynchronize(stream()));
-        }
+        art_run(h, method, b, nk, Kin, x0, nullptr, order, nord, opts, uniforms, nu, Xout, nullptr, nullptr);
+    });
+}
+
+extern "C" int fair_art_oracle(fair_csr *h, int32_t method, const double *b, int32_t nk, const int32_t *Kin,
+                               const double *x0, const double *xref, const int32_t *order, int32_t nord,
+                               const fair_opts *opts, const double *uniforms, int64_t nu, double *Xout,
+                               int32_t *exit_iter, int64_t *consumed) {
+    return guard([&] {
+        FAIR_CHECK(xref != nullptr, FAIR_ERR_ARG, "null xref");
+        art_run(h, method, b, nk, Kin, x0, xref, order, nord, opts, uniforms, nu, Xout, exit_iter, consumed);
     });
 }

@@ -93,8 +93,10 @@ contains
         end select
     end function art_code
 
-    ! art, AIRMs.f90:59-369.  With xref, iterations run one at a time and stop when the
-    ! distance to xref grows (AIRMs.f90:322, 330-333).
+    ! art, AIRMs.f90:59-369.  With xref the device stops once the distance to xref grows
+    ! (AIRMs.f90:322, 330-333); the random_number draws of randomized Kaczmarz are then taken as
+    ! twin_art takes them: an upper bound drawn from a saved generator state, the state restored
+    ! and advanced by exactly the number consumed.
     subroutine art(Xout, art_method, A, b, Kin, x0, xref, order, options)
         character(*) :: art_method
         class(csr_matrix), intent(in) :: A
@@ -107,14 +109,13 @@ contains
         type(fair_opts_c) :: o
         type(c_ptr) :: h, px0, pord, pu
         logical :: temp
-        integer(c_int32_t) :: meth, nord
+        integer(c_int32_t) :: meth, nord, ex
         integer(c_int32_t), allocatable, target :: ordc(:)
         integer(c_int32_t), allocatable :: kc(:)
-        integer(c_int64_t) :: cnt
+        integer(c_int64_t) :: cnt, used
         real(c_double), allocatable, target :: u(:), xs(:)
-        real(kind=8), allocatable :: xp(:), xn(:)
-        integer(c_int64_t) :: i
-        integer :: iter, maxiter, l, nperit, nsetup
+        integer, allocatable :: state(:)
+        integer :: ns
         meth = art_code(art_method)
         o = to_c(options)
         call borrow_dev(A, h, temp)
@@ -125,71 +126,29 @@ contains
             nord = size(order)
             pord = c_loc(ordc)
         end if
-        Xout = 0d0
+        kc = Kin
+        call fair_check(c_fair_art_uniform_count(h, meth, int(size(Kin), c_int32_t), kc, pord, nord, o, cnt))
+        allocate (u(max(cnt, 1_c_int64_t)))
+        if (present(xref)) then
+            call random_seed(size=ns)
+            allocate (state(ns))
+            call random_seed(get=state)
+        end if
+        if (cnt > 0) call random_number(u(1:cnt))
+        pu = c_null_ptr
+        if (cnt > 0) pu = c_loc(u)
+        px0 = c_null_ptr
+        if (present(x0)) then
+            xs = x0
+            px0 = c_loc(xs)
+        end if
         if (.not. present(xref)) then
-            kc = Kin
-            call fair_check(c_fair_art_uniform_count(h, meth, int(size(Kin), c_int32_t), kc, pord, nord, o, cnt))
-            allocate (u(max(cnt, 1_c_int64_t)))
-            do i = 1, cnt
-                call random_number(u(i))
-            end do
-            pu = c_null_ptr
-            if (cnt > 0) pu = c_loc(u)
-            px0 = c_null_ptr
-            if (present(x0)) then
-                xs = x0
-                px0 = c_loc(xs)
-            end if
             call fair_check(c_fair_art(h, meth, b, int(size(Kin), c_int32_t), kc, px0, pord, nord, o, pu, cnt, Xout))
         else
-            ! one iteration per call; the set-up scramble draws are consumed once (its result is
-            ! overwritten by the first iteration's scramble, AIRMs.f90:297/320)
-            kc = [1]
-            if (meth == 1) kc = [2]
-            call fair_check(c_fair_art_uniform_count(h, meth, 1_c_int32_t, kc, pord, nord, o, cnt))
-            nsetup = 0
-            nperit = 0
-            if (meth == 2) then
-                ! cnt = setup + one iteration; setup = 2*nord'
-                call fair_check(c_fair_art_uniform_count(h, meth, 1_c_int32_t, [0_c_int32_t], pord, nord, o, i))
-                nsetup = int(i)
-                nperit = int(cnt) - nsetup
-            end if
-            allocate (u(max(cnt, 1_c_int64_t)))
-            do i = 1, nsetup
-                call random_number(u(i))
-            end do
-            allocate (xs(A%n))
-            if (present(x0)) then
-                xs = x0
-            else
-                xs = 0d0
-            end if
-            maxiter = maxval(Kin)
-            if (meth == 1) maxiter = maxval(Kin)/2
-            l = 1
-            do iter = 1, maxiter
-                do i = nsetup + 1, nsetup + nperit
-                    call random_number(u(i))
-                end do
-                xp = xs
-                pu = c_null_ptr
-                if (cnt > 0) pu = c_loc(u)
-                allocate (xn(A%n))
-                call fair_check(c_fair_art(h, meth, b, 1_c_int32_t, kc, c_loc(xs), pord, nord, o, pu, cnt, xn))
-                xs = xn
-                deallocate (xn)
-                if (norm2(xs - xref) > norm2(xp - xref)) then
-                    Xout(:, size(Kin)) = xp
-                    exit
-                end if
-                if (l <= size(Kin)) then
-                    if ((meth == 1 .and. 2*iter == Kin(l)) .or. (meth /= 1 .and. iter == Kin(l))) then
-                        Xout(:, l) = xs
-                        l = l + 1
-                    end if
-                end if
-            end do
+            call fair_check(c_fair_art_oracle(h, meth, b, int(size(Kin), c_int32_t), kc, px0, xref, pord, nord, o, &
+                                              pu, cnt, Xout, ex, used))
+            call random_seed(put=state)
+            if (used > 0) call random_number(u(1:used))
         end if
         call release_dev(h, temp)
     end subroutine art

@@ -13,7 +13,8 @@ module fair_capi
               c_fair_transvecmul, c_fair_rowvecmult, c_fair_row_norms, c_fair_column_norms, &
               c_fair_transpose, c_fair_multiply_by_scalar, c_fair_add_scalar_mult_row, &
               c_fair_sirt, c_fair_normest, c_fair_gradient_descent, c_fair_art, &
-              c_fair_art_uniform_count, c_fair_linspace, c_fair_twin_art, c_fair_twin_art_uniform_bound
+              c_fair_art_uniform_count, c_fair_linspace, c_fair_twin_art, c_fair_twin_art_uniform_bound, &
+              c_fair_art_oracle
 
     ! bind(C) mirror of type(data) without its allocatables (system_generation.f90:6-12)
     type, bind(C) :: fair_geom_c
@@ -190,6 +191,21 @@ module fair_capi
             type(fair_opts_c), intent(in) :: opts
             integer(c_int64_t), value :: nu
             real(c_double) :: Xout(*)
+            integer(c_int) :: s
+        end function
+        function c_fair_art_oracle(h, method, b, nk, kin, x0, xref, order, nord, opts, u, nu, Xout, exit_iter, &
+                                   consumed) bind(C, name='fair_art_oracle') result(s)
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr, fair_opts_c
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: method, nk, nord
+            real(c_double), intent(in) :: b(*), xref(*)
+            integer(c_int32_t), intent(in) :: kin(*)
+            type(c_ptr), value :: x0, order, u
+            type(fair_opts_c), intent(in) :: opts
+            integer(c_int64_t), value :: nu
+            real(c_double) :: Xout(*)
+            integer(c_int32_t) :: exit_iter
+            integer(c_int64_t) :: consumed
             integer(c_int) :: s
         end function
         function c_fair_twin_art(h, method, stoprule, b, nk, kin, x0, y0, order, nord, slack, opts, u, nu, &

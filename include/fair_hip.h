@@ -171,12 +171,21 @@ int fair_normest(fair_csr *A, const double *v0, double tol, int32_t maxit, doubl
 int fair_gradient_descent(fair_csr *A, const double *b, int32_t maxiter, const double *x0,
                           double rho, double tol, double sigma, const fair_opts *opts,
                           double *xout, int32_t *iters);
-/* art, AIRMs.f90:59-369 (without the xref early stop).  order may be NULL (natural order).
+/* art, AIRMs.f90:59-369 without xref (fair_art_oracle has it).  order may be NULL (natural order).
  * uniforms: the random_number draws in the reference's order (see fair_art_uniform_count);
  * may be NULL for the non-random methods.  Xout: nk x n, row k = iterate at Kin[k]. */
 int fair_art(fair_csr *A, int32_t method, const double *b, int32_t nk, const int32_t *Kin,
              const double *x0, const int32_t *order, int32_t nord, const fair_opts *opts,
              const double *uniforms, int64_t nu, double *Xout);
+/* art with xref, AIRMs.f90:322, 330-333: stops when ||x - xref|| grows over an iteration and
+ * puts the iterate before that iteration in row nk-1 of Xout (rows not reached are zero).
+ * *exit_iter (may be NULL): the iteration that grew the distance, 0 if none did (iterations
+ * counted as Kin/2 for symkaczmarz); *consumed (may be NULL): the draws used.  nu may be below
+ * fair_art_uniform_count's figure: the draws are checked as each iteration takes them. */
+int fair_art_oracle(fair_csr *A, int32_t method, const double *b, int32_t nk, const int32_t *Kin,
+                    const double *x0, const double *xref, const int32_t *order, int32_t nord,
+                    const fair_opts *opts, const double *uniforms, int64_t nu, double *Xout,
+                    int32_t *exit_iter, int64_t *consumed);
 /* Number of random_number draws art consumes for these arguments. */
 int fair_art_uniform_count(fair_csr *A, int32_t method, int32_t nk, const int32_t *Kin,
                            const int32_t *order, int32_t nord, const fair_opts *opts,

@@ -293,6 +293,7 @@ class Reference:
         L.ref_twin_art.argtypes = [i, i, i, i, _F64P, i, _I32P, i, i, i, _F64P, _F64P, i, i, i, d, d, d, d, i,
                                    _F64P, C.POINTER(i), _F64P, _F64P, _F64P, i]
         L.ref_store.argtypes = [i, C.c_char_p, i]
+        L.ref_art_xref.argtypes = [i, i, i, _F64P, i, _I32P, i, i, _F64P, _F64P, i, i, d, d, d, d, i, _F64P]
         L.ref_rng_seed.argtypes = [i]
         L.ref_rng_draw.argtypes = [i, _F64P]
         L.ref_shepplogan.argtypes = [i, _F64P]
@@ -416,6 +417,19 @@ class Reference:
                        0.0 if ub is None else ub, 1.0 if lb is None else lb, relax, damp,
                        bool(rowprob), out)
         return out  # row k = iterate recorded at kin[k] (Fortran column k)
+
+    def art_xref(self, slot, method, b, kin, xref, x0=None, lb=None, ub=None, relax=1.0, damp=0.0,
+                 rowprob=False):
+        """art with the xref early stop; rows of the result not reached stay zero."""
+        m, n, _ = self.info(slot)
+        kin = np.ascontiguousarray(kin, np.int32)
+        out = np.zeros((len(kin), n))
+        x0a = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, np.float64)
+        self.L.ref_art_xref(slot, {"kaczmarz": 0, "symkaczmarz": 1, "randkaczmarz": 2}[method], m,
+                            np.ascontiguousarray(b, np.float64), len(kin), kin, x0 is not None, n, x0a,
+                            np.ascontiguousarray(xref, np.float64), ub is not None, lb is not None,
+                            0.0 if ub is None else ub, 1.0 if lb is None else lb, relax, damp, bool(rowprob), out)
+        return out
 
     def store(self, slot, path):
         p = os.fsencode(path)

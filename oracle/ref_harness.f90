@@ -272,6 +272,35 @@ contains
         xout = xo
     end subroutine ref_art
 
+    ! art with xref (AIRMs.f90:322, 330-333), natural row order.
+    subroutine ref_art_xref(slot, method, m, b, nk, kin, hasx0, n, x0, xref, upb, lwrb, ub, lb, relax, damp, &
+                            rowprob, xout) bind(C, name='ref_art_xref')
+        integer(c_int), value :: slot, method, m, nk, hasx0, n, upb, lwrb, rowprob
+        real(c_double), value :: ub, lb, relax, damp
+        real(c_double), intent(in) :: b(m), x0(n), xref(n)
+        integer(c_int), intent(in) :: kin(nk)
+        real(c_double), intent(out) :: xout(n, nk)
+        type(opts) :: options
+        character(len=12) :: meth
+        real(kind=8) :: xo(n, nk)
+        call fill_opts(options, upb, lwrb, ub, lb, relax, damp, rowprob)
+        select case (method)
+        case (0)
+            meth = 'kaczmarz'
+        case (1)
+            meth = 'symkaczmarz'
+        case default
+            meth = 'randkaczmarz'
+        end select
+        xo = 0d0
+        if (hasx0 /= 0) then
+            call art(xo, trim(meth), mats(slot), b, kin, x0=x0, xref=xref, options=options)
+        else
+            call art(xo, trim(meth), mats(slot), b, kin, xref=xref, options=options)
+        end if
+        xout = xo
+    end subroutine ref_art_xref
+
     ! twin_art (AIRMs.f90:371-811); stoprule 0 = 'errorgauge', 1 = 'mutualstep'.
     subroutine ref_twin_art(slot, method, stoprule, m, b, nk, kin, hasx0, hasy0, n, x0, y0, slack, &
                             upb, lwrb, ub, lb, relax, damp, rowprob, xout, final_it, xs, ys, eg, neg) &

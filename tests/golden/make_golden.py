@@ -25,7 +25,10 @@ Outputs (numpy .npz, inputs and expected outputs only):
   ref_image_small.txt, order_small.txt (--data) read_ref / read_ordering inputs, their results
                      in fan_data_small.npz
 
-usage: python tests/golden/make_golden.py [--c2 | --twin | --dump | --data]
+  art_oracle_small.npz (--artoracle) art with xref (the early stop of AIRMs.f90:322, 330-333)
+                     on a noisy small fan problem, with the uniform streams consumed
+
+usage: python tests/golden/make_golden.py [--c2 | --twin | --dump | --data | --artoracle]
 """
 from __future__ import annotations
 
@@ -291,6 +294,55 @@ def write_data(ref):
     np.savez_compressed(os.path.join(HERE, "fan_data_small.npz"), **d)
 
 
+def write_art_oracle(ref):
+    g = cf.Geometry(n=16, p=21, nth=30, deg_step=12.0)
+    th = g.theta()
+    IA, JA, vA = build(ref, 0, g.beam, g.n, g.p, g.nth, g.r, g.dw, g.sd, th)
+    xt = cf.shepplogan(g.n)
+    nord = int(np.count_nonzero(np.diff(IA)))
+    d = {"IA": IA, "JA": JA, "vA": vA, "xt": xt}
+    x0 = np.full(g.ncols, 0.05)
+    cases = [  # method, kin, eta, rowprob, x0, options
+        ("kaczmarz", [2, 5, 40], 0.1, False, None, {}),
+        ("kaczmarz", [2, 4, 40], 0.1, False, None, {}),          # the stop comes where 4 would be recorded
+        ("kaczmarz", [3, 10], 0.02, False, None, {"lb": 0.0}),   # no stop within 10
+        ("symkaczmarz", [4, 60], 0.1, False, x0, {"lb": 0.0, "ub": 1.0}),
+        ("randkaczmarz", [2, 30], 0.1, False, None, {"lb": 0.0}),
+        ("randkaczmarz", [30], 0.1, True, None, {"ub": 1.0, "relax": 0.8}),
+    ]
+    for i, (meth, kin, eta, rp, xs, kw) in enumerate(cases):
+        b = ref.matvec(0, xt) if eta == 0 else cf.add_noise(ref.matvec(0, xt), eta, 7)
+        ref.rng_seed(300 + i)
+        X = ref.art_xref(0, meth, b, kin, xt, x0=xs, rowprob=rp, **kw)
+        # the iteration that grew the distance, from the iterates of a run without xref
+        half = 2 if meth == "symkaczmarz" else 1
+        maxit = max(kin) // half
+        ref.rng_seed(300 + i)
+        full = ref.art(0, meth, b, [half * k for k in range(1, maxit + 1)], x0=xs, rowprob=rp, **kw)
+        prev = np.zeros(g.ncols) if xs is None else xs
+        stop = 0
+        for k in range(maxit):
+            if np.linalg.norm(full[k] - xt) > np.linalg.norm(prev - xt):
+                stop = k + 1
+                break
+            prev = full[k]
+        if stop:
+            assert np.array_equal(X[-1], prev), f"case {i}: early-stop row does not match"
+        used = 0
+        if meth == "randkaczmarz":
+            used = 2 * nord + (stop if stop else maxit) * (2 * nord + (nord if rp else 0))
+        ref.rng_seed(300 + i)
+        u = ref.rng_draw(used)
+        print(f"art xref case {i}: {meth} kin={kin} stop at {stop} (0 = none), {used} draws")
+        pre = f"case{i}/"
+        d.update({pre + "method": np.array(meth), pre + "kin": np.array(kin, np.int32), pre + "b": b,
+                  pre + "rowprob": np.array(rp), pre + "x0": np.zeros(0) if xs is None else xs,
+                  pre + "lb": np.array(kw.get("lb", np.nan)), pre + "ub": np.array(kw.get("ub", np.nan)),
+                  pre + "relax": np.array(kw.get("relax", 1.0)), pre + "u": u, pre + "stop": np.array(stop),
+                  pre + "X": X})
+    np.savez_compressed(os.path.join(HERE, "art_oracle_small.npz"), **d)
+
+
 def write_c2(ref):
     g = cf.C2
     IA, JA, vA, xt, b = problem(ref, g, cf.grains, 0.005, 2)
@@ -319,6 +371,9 @@ def main():
         return
     if "--data" in sys.argv:
         write_data(ref)
+        return
+    if "--artoracle" in sys.argv:
+        write_art_oracle(ref)
         return
     write_small(ref)
     write_c1(ref)
