@@ -236,7 +236,7 @@ void ensure_blocked(fair_csr *h) {
                            env_int("FAIR_LDS_A", 6144));
     ensure_T(h);
     if (!h->T.blk.ok) {
-        const int tx = env_int("FAIR_TX", 8), ty = env_int("FAIR_TY", 8);
+        const int tx = env_int("FAIR_TX", 16), ty = env_int("FAIR_TY", 8);
         build_blocked_tiles(h->T, h->g_n, h->g_p, h->g_nth, tx, ty, env_int("FAIR_LDS_T", 6144));
     }
     if (env_int("FAIR_DEBUG", 0) && built)

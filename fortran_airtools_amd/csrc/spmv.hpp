@@ -115,12 +115,23 @@ struct StagedChunk {
         }
     }
 
+    // Every LDS slot read here is valid (a pair's slots are below the layout's LDS capacity, also
+    // for the entry before an odd row start and past the row end), so the reads are issued
+    // together; entries outside the row are then dropped by selects, which keeps each lane's sums
+    // exactly those of its own entries, in order.
     __device__ __forceinline__ void use(int base, int s, int e, const V *lx, V &acc0, V &acc1) const {
+        V a[SPMV_UNROLL], b[SPMV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SPMV_UNROLL; ++u) {
+            a[u] = lx[l[u] & 0xffffu];
+            b[u] = lx[l[u] >> 16];
+        }
 #pragma unroll
         for (int u = 0; u < SPMV_UNROLL; ++u) {
             const int k = base + 2 * WAVE * u;
-            if (k >= s && k < e) acc0 += v[u].x * lx[l[u] & 0xffffu];
-            if (k + 1 < e) acc1 += v[u].y * lx[l[u] >> 16];
+            const V t0 = v[u].x * a[u], t1 = v[u].y * b[u];
+            acc0 = (k >= s && k < e) ? acc0 + t0 : acc0;
+            acc1 = (k + 1 < e) ? acc1 + t1 : acc1;
         }
     }
 };
@@ -152,6 +163,40 @@ __global__ void __launch_bounds__(256) k_spmv_wave(const int4 *__restrict__ rows
 // so the matrix stream and the footprint loads are in flight together.
 constexpr int SPMV_BLOCK_THREADS = 512;
 
+// Copies a block's footprint (segments of at most SEG_CHUNK = 16 contiguous columns) from x into
+// LDS.  Each wavefront takes 64 segments per round, one descriptor per lane (one coalesced load),
+// and copies them four at a time: 16 consecutive lanes per segment, so a load instruction touches
+// a few cache lines instead of one line per lane (the L1 tag lookups are what bounds the
+// footprint gathers).  Eight such loads are in flight before their LDS stores.
+template <class V>
+__device__ __forceinline__ void stage_footprint(const int4 bi, const int2 *__restrict__ segs,
+                                                const V *__restrict__ x, V *lx) {
+    static_assert(SEG_CHUNK == 16, "16 lanes copy one segment");
+    constexpr int NW = SPMV_BLOCK_THREADS / WAVE, PHASE = 8;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & (WAVE - 1);
+    const int sub = lane & 15, grp = lane >> 4;
+    for (int s0 = bi.z + wv * WAVE; s0 < bi.w; s0 += NW * WAVE) {
+        const int2 mine = s0 + lane < bi.w ? segs[s0 + lane] : make_int2(0, 0);
+        const int n = min(WAVE, bi.w - s0);             // segments of this round (wave-uniform)
+        for (int k0 = 0; 4 * k0 < n; k0 += PHASE) {
+            V v[PHASE];
+            int dst[PHASE];
+#pragma unroll
+            for (int k = 0; k < PHASE; ++k) {
+                const int j = 4 * (k0 + k) + grp;       // segment of this lane group
+                const int cx = __shfl(mine.x, j & (WAVE - 1), WAVE);
+                const int cy = __shfl(mine.y, j & (WAVE - 1), WAVE);
+                const bool on = j < n && sub < (cy >> 16);
+                dst[k] = on ? (cy & 0xffff) + sub : -1;
+                if (on) v[k] = x[cx + sub];
+            }
+#pragma unroll
+            for (int k = 0; k < PHASE; ++k)
+                if (dst[k] >= 0) lx[dst[k]] = v[k];
+        }
+    }
+}
+
 // One block of rows (blocked layout) by a workgroup of SPMV_BLOCK_THREADS; lx: LDS of the
 // block's footprint (zero_slot + 1 entries).  Shared by the single-matrix kernel and the grouped
 // one (group.hip), so both compute every row with the same operations in the same order.
@@ -171,14 +216,7 @@ __device__ __forceinline__ void spmv_block(const int4 bi, const int4 *__restrict
             ri = rows[q];
             ch.load((ri.y & ~1) + 2 * lane, ri.z, loc, vA);
         }
-        for (int s = bi.z + (int)threadIdx.x; s < bi.w; s += SPMV_BLOCK_THREADS) {
-            const int2 sg = segs[s];
-            const int len = sg.y >> 16;
-            const V *__restrict__ xs = x + sg.x;
-            V *d = lx + (sg.y & 0xffff);
-#pragma unroll 4
-            for (int t = 0; t < len; ++t) d[t] = xs[t];
-        }
+        stage_footprint<V>(bi, segs, x, lx);
         if (threadIdx.x == 0) lx[zero_slot] = (V)0;
         __syncthreads();
         bool first = true;
