@@ -558,7 +558,8 @@ def bench_uneven(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     e1.record(stream)
     torch.cuda.synchronize()
     t_group = e0.elapsed_time(e1) / 1e3
-    log(f"group: {iters} iterations {t_group * 1e3:.1f} ms ({wg} workgroups per CU), set-up {t_setup:.2f} s")
+    grid = "one workgroup per item" if wg == 0 else f"{wg} workgroups per CU"
+    log(f"group: {iters} iterations {t_group * 1e3:.1f} ms ({grid}), set-up {t_setup:.2f} s")
     xs = [torch.zeros(A.n, dtype=torch.float64, device=dev) for A in mats]
     for A, b, x in zip(mats, db, xs):       # weights, layouts and workspaces, untimed
         airms.sirt_dev("sart", A, b.data_ptr(), 1, x.data_ptr(), o)

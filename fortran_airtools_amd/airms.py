@@ -263,6 +263,8 @@ class SirtGroup:
             pass
 
     def autotune(self) -> int:
+        """Times the grid choices and keeps the fastest: workgroups per CU striding over the work
+        items, or 0 for one workgroup per item.  Returns the choice."""
         c = C.c_int32()
         check(lib().fair_group_autotune(self.handle, C.byref(c)))
         return c.value

@@ -97,7 +97,7 @@ __device__ __forceinline__ void group_item(const int4 itm, const GMat &g, const 
 }
 
 template <int PASS>
-__global__ void __launch_bounds__(SPMV_BLOCK_THREADS) k_group_pass(const int4 *__restrict__ items, int64_t nitems,
+__global__ void __launch_bounds__(SPMV_BLOCK_THREADS, 6) k_group_pass(const int4 *__restrict__ items, int64_t nitems,
                                                                    const GroupProb *__restrict__ P, double relax,
                                                                    GBox box) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -147,13 +147,18 @@ void group_iterate(fair_group *g, int maxits, const fair_opts *o) {
         box.ub = o->ubound;
         box.lb = o->lbound;
     }
-    const unsigned grid = (unsigned)std::max(1, g->ncu * g->wg_per_cu);
+    // wg_per_cu workgroups per CU stride over a class's items; 0: one workgroup per item, left to
+    // the hardware dispatcher
+    auto grid_of = [&](const fair_group_class &c) {
+        const int64_t g_items = std::max<int64_t>(1, c.n);
+        return (unsigned)(g->wg_per_cu > 0 ? std::min<int64_t>(g_items, (int64_t)g->ncu * g->wg_per_cu) : g_items);
+    };
     for (int it = 0; it < maxits; ++it) {
         for (const fair_group_class &c : g->passA)
-            hipLaunchKernelGGL(k_group_pass<0>, dim3(grid), dim3(SPMV_BLOCK_THREADS), c.lds_bytes, stream(),
+            hipLaunchKernelGGL(k_group_pass<0>, dim3(grid_of(c)), dim3(SPMV_BLOCK_THREADS), c.lds_bytes, stream(),
                                (const int4 *)c.items.p, c.n, g->probs.p, relax, box);
         for (const fair_group_class &c : g->passT)
-            hipLaunchKernelGGL(k_group_pass<1>, dim3(grid), dim3(SPMV_BLOCK_THREADS), c.lds_bytes, stream(),
+            hipLaunchKernelGGL(k_group_pass<1>, dim3(grid_of(c)), dim3(SPMV_BLOCK_THREADS), c.lds_bytes, stream(),
                                (const int4 *)c.items.p, c.n, g->probs.p, relax, box);
     }
     launch_check("group sirt");
@@ -301,7 +306,7 @@ extern "C" int fair_group_autotune(fair_group *g, int32_t *wg_per_cu) {
         FAIR_HIP(hipEventCreate(&e1));
         int best = g->wg_per_cu;
         float best_ms = 1e30f;
-        for (int c : {1, 2, 3, 4}) {
+        for (int c : {0, 1, 2, 3, 4}) {
             g->wg_per_cu = c;
             group_iterate(g, 1, nullptr);   // warm-up
             FAIR_HIP(hipEventRecord(e0, stream()));

@@ -218,7 +218,8 @@ int fair_group_free(fair_group *G);
 /* maxits iterations of every problem: d_b[p] (m_p), d_x[p] (n_p; start in, result out) in HBM. */
 int fair_group_sirt_dev(fair_group *G, const double *const *d_b, int32_t maxits, double *const *d_x,
                         const fair_opts *opts);
-/* Picks the persistent grid size (workgroups per CU) by timing; wg_per_cu may be NULL. */
+/* Picks the grid by timing: wg_per_cu workgroups per CU striding over the work items, or 0 for
+ * one workgroup per item; wg_per_cu (may be NULL) receives the choice. */
 int fair_group_autotune(fair_group *G, int32_t *wg_per_cu);
 
 /* ---- matrix-free fan-beam projector (FP32) ------------------------------------------------
