@@ -212,6 +212,44 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// The same sum through DPP lane moves in the vector ALU instead of LDS permutes: within each row
+// of 16 lanes, lane i adds lane i^1, then i^2, then the mirror within 8 and within 16 (every lane
+// of the row then holds the row sum, bit-identical across the row since the adds pair the same
+// operands), and the four row sums are read as scalars and added as (r0 + r1) + (r2 + r3).  All
+// 64 lanes must be active.  The summation order differs from wave_sum's.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float lane_value(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum_dpp(T v) {
+    v += dpp_move<0xB1>(v);    // quad_perm [1,0,3,2]: lane i^1
+    v += dpp_move<0x4E>(v);    // quad_perm [2,3,0,1]: lane i^2
+    v += dpp_move<0x141>(v);   // row_half_mirror: the other quad of the 8
+    v += dpp_move<0x140>(v);   // row_mirror: the other 8 of the 16
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
 // Fortran MIN / MAX on reals as the reference compiler evaluates them:
 // min(a,b) = a<b ? a : b, max(a,b) = a>b ? a : b (a NaN `a` yields b).
 __host__ __device__ __forceinline__ double fmin_ref(double a, double b) { return a < b ? a : b; }

@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, con
             const int j = JA[k];
             if ((uint32_t)j < (uint32_t)ncols) dot += x[j] * vA[k];
         }
-        dot = wave_sum(dot);
+        dot = wave_sum_dpp(dot);
         const double coeff = (relax * (cur.b - dot)) / cur.w;
 #pragma unroll
         for (int u = 0; u < ART_R; ++u) {

@@ -89,7 +89,7 @@ __device__ __forceinline__ V row_direct(int s, int e, int lane, const int32_t *_
             if (k + 1 < e) acc1 += v[u].y * gather(x, c[u].y, ncols);
         }
     }
-    return wave_sum(acc0 + acc1);
+    return wave_sum_dpp(acc0 + acc1);
 }
 
 // One chunk (2 * 64 * SPMV_UNROLL entries) of a row held in registers: values and 16-bit
@@ -233,7 +233,7 @@ __device__ __forceinline__ void spmv_block(const int4 bi, const int4 *__restrict
                 ch.load(base, ri.z, loc, vA);
                 ch.use(base, ri.y, ri.z, lx, acc0, acc1);
             }
-            const V acc = wave_sum(acc0 + acc1);
+            const V acc = wave_sum_dpp(acc0 + acc1);
             if (lane == 0) epi(ri.x, acc);
         }
     } else {
