@@ -74,6 +74,8 @@ struct fair_mf {
     int32_t n_exact = 0;
     int64_t ex_segments = 0;    // table entries of positive length
     fair::DBuf<uint8_t> flag;   // per ray: 1 = in the table (skipped by the traversal kernels)
+    // forward workspace: x transposed (see k_mf_transpose); calls on one handle must not overlap
+    fair::DBuf<float> xT;
 };
 
 namespace fair {
@@ -192,14 +194,38 @@ struct Walk {
     }
 };
 
+// xT[ry * n + ix] = x[ix * n + ry] (ry = n - 1 - iy counts down y, as the columns of A do).
+__global__ void __launch_bounds__(256) k_mf_transpose(const float *__restrict__ x, float *__restrict__ xT, int n) {
+    __shared__ float t[32][33];
+    const int rb = blockIdx.x * 32, ib = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int ix = ib + k, ry = rb + tx;
+        if (ix < n && ry < n) t[k][tx] = x[(int64_t)ix * n + ry];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int ry = rb + k, ix = ib + tx;
+        if (ix < n && ry < n) xT[(int64_t)ry * n + ix] = t[tx][k];
+    }
+}
+
+// One thread per ray.  Neighbouring lanes hold neighbouring rays of one angle, which at a given
+// step sit one pixel apart across the ray: for rays running mostly along x that is the next
+// column index of x, for rays running mostly along y the next row, n entries away.  Those lanes
+// gather from xT instead, so that a gather instruction touches a few cache lines rather than one
+// per lane.  The values and the order of the sums are the same either way.
 __global__ void __launch_bounds__(256) k_mf_forward(MfArgs g, int64_t m, const float *__restrict__ x,
-                                                    const float *__restrict__ b, float *__restrict__ y) {
+                                                    const float *__restrict__ xT, const float *__restrict__ b,
+                                                    float *__restrict__ y) {
     const int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ray >= m || g.flag[ray]) return;
     const int i = (int)(ray / g.p), j = (int)(ray - (int64_t)i * g.p);
     double x0, y0, a, bb;
     ray_line(g, i, j, x0, y0, a, bb);
     const int n = g.n;
+    const bool ymajor = fabs(bb) > fabs(a);
+    const float *__restrict__ src = ymajor ? xT : x;
     Walk w;
     w.init(x0, y0, a, bb, n, 0, n, 0, n);
     float acc = 0.f;
@@ -213,11 +239,11 @@ __global__ void __launch_bounds__(256) k_mf_forward(MfArgs g, int64_t m, const f
             if (w.live) {
                 int ix, iy;
                 w.next(ix, iy, sg[k]);
-                pix[k] = ix * n + (n - 1 - iy);
+                pix[k] = ymajor ? (n - 1 - iy) * n + ix : ix * n + (n - 1 - iy);
             }
         }
 #pragma unroll
-        for (int k = 0; k < MF_BATCH; ++k) acc = fmaf(sg[k], x[pix[k]], acc);
+        for (int k = 0; k < MF_BATCH; ++k) acc = fmaf(sg[k], src[pix[k]], acc);
     }
     y[ray] = b ? acc - b[ray] : acc;
 }
@@ -462,9 +488,13 @@ MfArgs args(const fair_mf *h) {
     return a;
 }
 
-void mf_forward(const fair_mf *h, const float *x, const float *b, float *y) {
-    if (h->m > 0)
-        hipLaunchKernelGGL(k_mf_forward, dim3(grid_for(h->m, 256)), dim3(256), 0, stream(), args(h), h->m, x, b, y);
+void mf_forward(fair_mf *h, const float *x, const float *b, float *y) {
+    if (h->m > 0) {
+        const unsigned nb = (unsigned)((h->n + 31) / 32);
+        hipLaunchKernelGGL(k_mf_transpose, dim3(nb, nb), dim3(256), 0, stream(), x, h->xT.p, h->n);
+        hipLaunchKernelGGL(k_mf_forward, dim3(grid_for(h->m, 256)), dim3(256), 0, stream(), args(h), h->m, x, h->xT.p,
+                           b, y);
+    }
     if (h->n_exact > 0)
         hipLaunchKernelGGL(k_mf_table_forward, dim3(grid_for(h->n_exact, 64)), dim3(64), 0, stream(), h->ex_rays.p,
                            h->ex_offs.p, h->ex_cols.p, h->ex_vals.p, h->n_exact, x, b, y);
@@ -582,6 +612,7 @@ extern "C" int fair_mf_create(const fair_geom *g, const double *theta, fair_mf *
         FAIR_CHECK(h->ecap >= 24, FAIR_ERR_OVERFLOW, "too many rays for the fixed-point adjoint");
         const int64_t nn = (int64_t)n * n;
         h->xacc.alloc(nn);
+        h->xT.alloc(nn);
         h->rmax.alloc(1);
         FAIR_HIP(hipMemsetAsync(h->xacc.p, 0, sizeof(unsigned long long) * nn, stream()));
         FAIR_HIP(hipMemsetAsync(h->rmax.p, 0, sizeof(unsigned), stream()));
