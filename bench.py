@@ -108,8 +108,8 @@ def main() -> int:
                     help="also run a 1 GiB fill and a 1 GiB sum (reference kernels for HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              f"(asked for --gpus {a.gpus}).", file=sys.stderr)
+        print("bench.py measures one GPU only: the library drives a single device and has no "
+              f"multi-GPU path (asked for --gpus {a.gpus}).", file=sys.stderr)
         return 2
 
     import numpy as np
