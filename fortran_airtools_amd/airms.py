@@ -201,6 +201,52 @@ def art(art_method: str, A: CSRMatrix, b, Kin, x0=None, order=None, options: Opt
     return out
 
 
+def basic_sweep(x, A: CSRMatrix, b, ord_, nord: int, isrand: bool, rowprob: bool, omg: float, nrA, cumul,
+                upb: bool, lwrb: bool, ub: float, lb: float, uniforms=None, seed: int = 0) -> np.ndarray:
+    """call basic_sweep(x, A, b, ord, nord, isrand, rowprob, omg, nrA, cumul, upb, lwrb, ub, lb),
+    AIRMs.f90:813-858.  Returns the updated x.  With isrand and rowprob the row of each step is the
+    first whose cumul reaches that step's RANDOM_NUMBER draw (`uniforms`, nord of them in the
+    reference's order; seeded draws when not supplied); otherwise the rows are ord(1:nord)."""
+    xs = np.array(_f64(x, A.n), copy=True)
+    b = _f64(b, A.m)
+    nrA = _f64(nrA, A.m)
+    if isrand and rowprob:
+        cumul = _f64(cumul, A.m)
+        u = np.random.default_rng(seed).random(nord) if uniforms is None else np.asarray(uniforms, np.float64)
+        if u.shape[0] < nord:
+            raise FairError(FAIR_ERR_ARG, "too few uniforms for the sweep")
+        rows = np.minimum(np.searchsorted(cumul, u[:nord], side="left") + 1, A.m).astype(np.int32)
+    else:
+        rows = np.ascontiguousarray(np.asarray(ord_, np.int32)[:nord])
+        if rows.shape[0] < nord:
+            raise FairError(FAIR_ERR_ARG, "ord holds fewer than nord rows")
+    o = Opts()
+    o.relaxpar = float(omg)
+    if upb:
+        o.set_ubound(ub)
+    if lwrb:
+        o.set_lbound(lb)
+    co = _opts(o)
+    check(lib().fair_art_steps(A.handle, _ptr(rows), int(nord), _ptr(b), _ptr(nrA), C.byref(co), _ptr(xs)))
+    return xs
+
+
+def art_steps_dev(A: CSRMatrix, d_rows: int, nsteps: int, d_b: int, d_nrA: int, omg: float, d_x: int,
+                  ub: float | None = None, lb: float | None = None) -> None:
+    """The row steps of basic_sweep (AIRMs.f90:813-858) with device addresses: int32 one-based rows
+    (nsteps, each in 1..m), b and nrA (m), x (n, updated in place).  Asynchronous on the library
+    stream."""
+    o = Opts()
+    o.relaxpar = float(omg)
+    if ub is not None:
+        o.set_ubound(ub)
+    if lb is not None:
+        o.set_lbound(lb)
+    co = _opts(o)
+    check(lib().fair_art_steps_dev(A.handle, C.c_void_p(d_rows), int(nsteps), C.c_void_p(d_b), C.c_void_p(d_nrA),
+                                   C.byref(co), C.c_void_p(d_x)))
+
+
 STOP_RULES = {"errorgauge": 0, "mutualstep": 1}
 
 

@@ -685,13 +685,43 @@ extern "C" int fair_twin_art_uniform_bound(fair_csr *h, int32_t method, int32_t 
     });
 }
 
+// basic_sweep, AIRMs.f90:813-858, with host arrays: the row steps over rows[0..nsteps) (one-based
+// rows, chosen by the caller), x updated in place.  Returns after x is copied back.
+extern "C" int fair_art_steps(fair_csr *h, const int32_t *rows, int64_t nsteps, const double *b, const double *nrA,
+                              const fair_opts *o, double *x) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(x && b && nrA && (rows || nsteps == 0) && nsteps >= 0, FAIR_ERR_ARG, "null argument");
+        const int32_t m = h->A.m, n = h->A.n;
+        for (int64_t s = 0; s < nsteps; ++s)
+            FAIR_CHECK(rows[s] >= 1 && rows[s] <= m, FAIR_ERR_ARG, "row index out of range in the sweep");
+        if (nsteps == 0 || n == 0) return;
+        DBuf<int32_t> dr((size_t)nsteps);
+        DBuf<double> db(m > 0 ? m : 1), dw(m > 0 ? m : 1), dx(n);
+        dr.upload(rows, (size_t)nsteps);
+        db.upload(b, m);
+        dw.upload(nrA, m);
+        dx.upload(x, n);
+        const Box box = make_box(o);
+        const double relax = o ? o->relaxpar : 1.0;
+        hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
+                           SweepChains{{dr.p, nullptr}, {nsteps, 0}, {dx.p, nullptr}}, h->A.IA.p, h->A.JA.p,
+                           h->A.vA.p, db.p, dw.p, n, relax, box.lwrb, box.lb, box.upb, box.ub);
+        launch_check("art sweep");
+        dx.download(x, n);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+    });
+}
+
 extern "C" int fair_art_steps_dev(fair_csr *h, const int32_t *d_rows, int64_t nsteps, const double *d_b,
                                   const double *d_nrA, const fair_opts *o, double *d_x) {
     return guard([&] {
         check_handle(h);
+        FAIR_CHECK(nsteps >= 0 && (nsteps == 0 || (d_rows && d_b && d_nrA && d_x)), FAIR_ERR_ARG,
+                   "null argument");
         const Box box = make_box(o);
         const double relax = o ? o->relaxpar : 1.0;
-        if (nsteps > 0)
+        if (nsteps > 0 && h->A.n > 0)
             hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
                                SweepChains{{d_rows, nullptr}, {nsteps, 0}, {d_x, nullptr}}, h->A.IA.p, h->A.JA.p,
                                h->A.vA.p, d_b, d_nrA, h->A.n, relax, box.lwrb, box.lb, box.upb, box.ub);

@@ -226,6 +226,57 @@ contains
         call release_dev(h, temp)
     end subroutine twin_art
 
+    ! basic_sweep, AIRMs.f90:813-858: the row steps over ord(1:nord), or with isrand and rowprob
+    ! one random_number draw per step and the first row whose cumul reaches it (a binary search
+    ! over the non-decreasing table finds the row the reference's linear search finds).  The rows
+    ! are chosen here, in the reference's draw order; the steps run on the device as one launch.
+    subroutine basic_sweep(x, A, b, ord, nord, isrand, rowprob, omg, nrA, cumul, upb, lwrb, ub, lb)
+        class(csr_matrix), intent(in) :: A
+        real(kind=8), intent(inout) :: x(A%n)
+        real(kind=8), intent(in) :: b(A%m), cumul(A%m), nrA(A%m), omg, ub, lb
+        integer, intent(in) :: ord(2*A%m), nord
+        logical, intent(in) :: isrand, rowprob, upb, lwrb
+        type(fair_opts_c) :: o
+        type(c_ptr) :: h
+        logical :: temp
+        integer(c_int32_t), allocatable :: rows(:)
+        real(c_double), allocatable :: xs(:)
+        real(kind=8) :: u
+        integer :: ii, lo, hi, mid
+        allocate (rows(max(nord, 1)))
+        do ii = 1, nord
+            if (isrand .and. rowprob) then
+                call random_number(u)
+                lo = 1
+                hi = A%m
+                do while (lo < hi)
+                    mid = (lo + hi)/2
+                    if (cumul(mid) < u) then
+                        lo = mid + 1
+                    else
+                        hi = mid
+                    end if
+                end do
+                rows(ii) = lo
+            else
+                rows(ii) = ord(ii)
+            end if
+        end do
+        o%relaxpar = omg
+        o%ubound = ub
+        o%lbound = lb
+        o%damp = 0d0
+        o%verbose = 0
+        o%upb = merge(1, 0, upb)
+        o%lwrb = merge(1, 0, lwrb)
+        o%rowprobability = 0
+        xs = x
+        call borrow_dev(A, h, temp)
+        call fair_check(c_fair_art_steps(h, rows, int(max(nord, 0), c_int64_t), b, nrA, o, xs))
+        call release_dev(h, temp)
+        x = xs
+    end subroutine basic_sweep
+
     ! sirt, AIRMs.f90:997-1127 ('cimmino' | 'sart'); SART leaves A's rows column-sorted.
     subroutine sirt(xout, sirt_method, A, b, Kin, x0, options)
         character(*) :: sirt_method

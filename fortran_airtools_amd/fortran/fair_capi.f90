@@ -14,7 +14,7 @@ module fair_capi
               c_fair_transpose, c_fair_multiply_by_scalar, c_fair_add_scalar_mult_row, &
               c_fair_sirt, c_fair_normest, c_fair_gradient_descent, c_fair_art, &
               c_fair_art_uniform_count, c_fair_linspace, c_fair_twin_art, c_fair_twin_art_uniform_bound, &
-              c_fair_art_oracle
+              c_fair_art_oracle, c_fair_art_steps
 
     ! bind(C) mirror of type(data) without its allocatables (system_generation.f90:6-12)
     type, bind(C) :: fair_geom_c
@@ -191,6 +191,16 @@ module fair_capi
             type(fair_opts_c), intent(in) :: opts
             integer(c_int64_t), value :: nu
             real(c_double) :: Xout(*)
+            integer(c_int) :: s
+        end function
+        function c_fair_art_steps(h, rows, nsteps, b, nrA, opts, x) bind(C, name='fair_art_steps') result(s)
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr, fair_opts_c
+            type(c_ptr), value :: h
+            integer(c_int32_t), intent(in) :: rows(*)
+            integer(c_int64_t), value :: nsteps
+            real(c_double), intent(in) :: b(*), nrA(*)
+            type(fair_opts_c), intent(in) :: opts
+            real(c_double) :: x(*)
             integer(c_int) :: s
         end function
         function c_fair_art_oracle(h, method, b, nk, kin, x0, xref, order, nord, opts, u, nu, Xout, exit_iter, &

@@ -41,6 +41,43 @@ contains
         g%sd = dat%sd
     end function geom_of
 
+    ! quicksort, system_generation.f90:37-77: Hoare partition around the middle element of
+    ! a(first:last), IND permuted alongside; the same comparisons and swaps, so ties end in the
+    ! same order (the sort is not stable).
+    recursive subroutine quicksort(a, IND, first, last)
+        real(kind=8) :: a(:)
+        integer :: IND(:)
+        integer, optional :: first, last
+        real(kind=8) :: pivot, t
+        integer :: i, j, lo, hi, k
+        lo = 1
+        hi = size(a)
+        if (present(first)) lo = first
+        if (present(last)) hi = last
+        pivot = a((lo + hi)/2)
+        i = lo
+        j = hi
+        do
+            do while (a(i) < pivot)
+                i = i + 1
+            end do
+            do while (pivot < a(j))
+                j = j - 1
+            end do
+            if (i >= j) exit
+            t = a(i)
+            a(i) = a(j)
+            a(j) = t
+            k = IND(i)
+            IND(i) = IND(j)
+            IND(j) = k
+            i = i + 1
+            j = j - 1
+        end do
+        if (lo < i - 1) call quicksort(a, IND, lo, i - 1)
+        if (j + 1 < hi) call quicksort(a, IND, j + 1, hi)
+    end subroutine quicksort
+
     ! fanlineartomo, system_generation.f90:79-307
     function fanlineartomo(dat) result(Amat)
         class(data), intent(in) :: dat

@@ -201,8 +201,13 @@ int fair_twin_art(fair_csr *A, int32_t method, int32_t stoprule, const double *b
                   int32_t *final_it, double *Xstore, double *Ystore, double *EG, int64_t *consumed);
 int fair_twin_art_uniform_bound(fair_csr *A, int32_t method, int32_t nk, const int32_t *Kin,
                                 const int32_t *order, int32_t nord, const fair_opts *opts, int64_t *count);
-/* basic_sweep's row steps, AIRMs.f90:840-856, over an explicit one-based row sequence, as one
- * persistent launch.  nrA: damped squared row norms (m).  d_x in HBM. */
+/* basic_sweep, AIRMs.f90:813-858: the row steps over an explicit one-based row sequence (the
+ * caller makes the row choice: ord, or the row-probability draws), as one persistent launch.
+ * nrA: the rows' (damped squared) norms (m); x (n) updated in place.  Host arrays; rows outside
+ * 1..m are rejected. */
+int fair_art_steps(fair_csr *A, const int32_t *rows, int64_t nsteps, const double *b, const double *nrA,
+                   const fair_opts *opts, double *x);
+/* The same with device arrays (d_x in HBM), asynchronous; the rows must lie in 1..m. */
 int fair_art_steps_dev(fair_csr *A, const int32_t *d_rows, int64_t nsteps, const double *d_b,
                        const double *d_nrA, const fair_opts *opts, double *d_x);
 

@@ -293,6 +293,8 @@ class Reference:
         L.ref_twin_art.argtypes = [i, i, i, i, _F64P, i, _I32P, i, i, i, _F64P, _F64P, i, i, i, d, d, d, d, i,
                                    _F64P, C.POINTER(i), _F64P, _F64P, _F64P, i]
         L.ref_store.argtypes = [i, C.c_char_p, i]
+        L.ref_basic_sweep.argtypes = [i, i, i, _F64P, _F64P, i, _I32P, i, i, d, _F64P, _F64P, i, i, d, d]
+        L.ref_quicksort.argtypes = [i, _F64P, _I32P]
         L.ref_art_xref.argtypes = [i, i, i, _F64P, i, _I32P, i, i, _F64P, _F64P, i, i, d, d, d, d, i, _F64P]
         L.ref_rng_seed.argtypes = [i]
         L.ref_rng_draw.argtypes = [i, _F64P]
@@ -417,6 +419,25 @@ class Reference:
                        0.0 if ub is None else ub, 1.0 if lb is None else lb, relax, damp,
                        bool(rowprob), out)
         return out  # row k = iterate recorded at kin[k] (Fortran column k)
+
+    def basic_sweep(self, slot, x, b, ord_, nord, nrA, cumul, omg=1.0, isrand=False, rowprob=False, lb=None,
+                    ub=None):
+        """basic_sweep on a slot's matrix; returns the updated x."""
+        m, n, _ = self.info(slot)
+        xv = np.array(x, np.float64, copy=True)
+        ordv = np.zeros(2 * m, np.int32)
+        ordv[:len(ord_)] = ord_
+        self.L.ref_basic_sweep(slot, m, n, xv, np.ascontiguousarray(b, np.float64), nord, ordv, bool(isrand),
+                               bool(rowprob), omg, np.ascontiguousarray(nrA, np.float64),
+                               np.ascontiguousarray(cumul, np.float64), ub is not None, lb is not None,
+                               0.0 if ub is None else ub, 0.0 if lb is None else lb)
+        return xv
+
+    def quicksort(self, a, ind=None):
+        a = np.array(a, np.float64, copy=True)
+        ind = np.arange(1, len(a) + 1, dtype=np.int32) if ind is None else np.array(ind, np.int32, copy=True)
+        self.L.ref_quicksort(len(a), a, ind)
+        return a, ind
 
     def art_xref(self, slot, method, b, kin, xref, x0=None, lb=None, ub=None, relax=1.0, damp=0.0,
                  rowprob=False):

@@ -272,6 +272,37 @@ contains
         xout = xo
     end subroutine ref_art
 
+    ! basic_sweep (AIRMs.f90:813-858) on a slot's matrix; x updated in place.
+    subroutine ref_basic_sweep(slot, m, n, x, b, nord, ord, isrand, rowprob, omg, nrA, cumul, upb, lwrb, ub, lb) &
+        bind(C, name='ref_basic_sweep')
+        integer(c_int), value :: slot, m, n, nord, isrand, rowprob, upb, lwrb
+        real(c_double), value :: omg, ub, lb
+        real(c_double), intent(inout) :: x(n)
+        real(c_double), intent(in) :: b(m), nrA(m), cumul(m)
+        integer(c_int), intent(in) :: ord(2*m)
+        integer :: ordv(2*m)
+        real(kind=8) :: xv(n)
+        ordv = ord
+        xv = x
+        call basic_sweep(xv, mats(slot), b, ordv, nord, isrand /= 0, rowprob /= 0, omg, nrA, cumul, &
+                         upb /= 0, lwrb /= 0, ub, lb)
+        x = xv
+    end subroutine ref_basic_sweep
+
+    ! quicksort (system_generation.f90:37-77) of a(1:nn) with IND alongside.
+    subroutine ref_quicksort(nn, a, ind) bind(C, name='ref_quicksort')
+        integer(c_int), value :: nn
+        real(c_double), intent(inout) :: a(nn)
+        integer(c_int), intent(inout) :: ind(nn)
+        real(kind=8) :: av(nn)
+        integer :: iv(nn)
+        av = a
+        iv = ind
+        call quicksort(av, iv)
+        a = av
+        ind = iv
+    end subroutine ref_quicksort
+
     ! art with xref (AIRMs.f90:322, 330-333), natural row order.
     subroutine ref_art_xref(slot, method, m, b, nk, kin, hasx0, n, x0, xref, upb, lwrb, ub, lb, relax, damp, &
                             rowprob, xout) bind(C, name='ref_art_xref')

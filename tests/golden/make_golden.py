@@ -28,7 +28,10 @@ Outputs (numpy .npz, inputs and expected outputs only):
   art_oracle_small.npz (--artoracle) art with xref (the early stop of AIRMs.f90:322, 330-333)
                      on a noisy small fan problem, with the uniform streams consumed
 
-usage: python tests/golden/make_golden.py [--c2 | --twin | --dump | --data | --artoracle]
+  sweep_small.npz    (--sweep) basic_sweep on a small fan problem (natural, reversed and
+                     row-probability orders, with and without a box) and quicksort of arrays with ties
+
+usage: python tests/golden/make_golden.py [--c2 | --twin | --dump | --data | --artoracle | --sweep]
 """
 from __future__ import annotations
 
@@ -343,6 +346,37 @@ def write_art_oracle(ref):
     np.savez_compressed(os.path.join(HERE, "art_oracle_small.npz"), **d)
 
 
+def write_sweep(ref):
+    g = cf.Geometry(n=16, p=21, nth=30, deg_step=12.0)
+    IA, JA, vA, xt, b = problem(ref, g, cf.shepplogan, 0.02, 9)
+    m, n = g.m, g.ncols
+    nrA = ref.row_norms(0, 2) ** 2
+    rows = (np.flatnonzero(nrA > 0) + 1).astype(np.int32)
+    cumul = np.ones(m)
+    cumul[:len(rows)] = np.cumsum(nrA[:len(rows)]) / nrA.sum()
+    d = {"IA": IA, "JA": JA, "vA": vA, "b": b, "nrA": nrA, "cumul": cumul}
+    x0 = np.random.default_rng(4).random(n) * 0.3
+    cases = [("natural", rows, False, 1.3, 0.0, 1.0), ("reversed", rows[::-1].copy(), False, 0.8, None, None),
+             ("rowprob", rows, True, 1.0, 0.0, None)]
+    for name, ordv, rp, omg, lb, ub in cases:
+        ref.rng_seed(77)
+        x = ref.basic_sweep(0, x0, b, ordv, len(ordv), nrA, cumul, omg=omg, isrand=rp, rowprob=rp, lb=lb, ub=ub)
+        ref.rng_seed(77)
+        d[f"{name}/u"] = ref.rng_draw(len(ordv) if rp else 0)
+        d[f"{name}/ord"] = ordv
+        d[f"{name}/omg"] = np.array(omg)
+        d[f"{name}/lb"] = np.array(np.nan if lb is None else lb)
+        d[f"{name}/ub"] = np.array(np.nan if ub is None else ub)
+        d[f"{name}/x"] = x
+    d["x0"] = x0
+    rng = np.random.default_rng(8)
+    for k, nn in enumerate((1, 2, 7, 100, 1000)):
+        a = np.round(rng.random(nn) * 20) / 4.0          # many ties
+        sa, si = ref.quicksort(a)
+        d[f"qs{k}/a"], d[f"qs{k}/sorted"], d[f"qs{k}/ind"] = a, sa, si
+    np.savez_compressed(os.path.join(HERE, "sweep_small.npz"), **d)
+
+
 def write_c2(ref):
     g = cf.C2
     IA, JA, vA, xt, b = problem(ref, g, cf.grains, 0.005, 2)
@@ -374,6 +408,9 @@ def main():
         return
     if "--artoracle" in sys.argv:
         write_art_oracle(ref)
+        return
+    if "--sweep" in sys.argv:
+        write_sweep(ref)
         return
     write_small(ref)
     write_c1(ref)
