@@ -94,13 +94,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
-    ap.add_argument("--mode", default="sart", choices=["sart", "precision", "batch", "matfree", "kaczmarz", "uneven"],
+    ap.add_argument("--mode", default="sart",
+                    choices=["sart", "precision", "batch", "matfree", "kaczmarz", "uneven", "phantoms"],
                     help="sart: the headline N=512 FP64 SART line (default); precision: FP32 vs FP64 "
                          "time to equal error at N=512; batch: 256 slices of 256^2 as one batched FP32 run; "
                          "matfree: matrix-free FP32 gradient descent at 2048^2 and at N=512 against the "
                          "FP64 CSR method; kaczmarz: randomized (row-norm sampling) and symmetric "
                          "Kaczmarz, 20 sweeps at N=256, against the oracle and the reference on the host; "
-                         "uneven: 64 SART problems from N=128 to N=1024 as one group against one run each")
+                         "uneven: 64 SART problems from N=128 to N=1024 as one group against one run each; "
+                         "phantoms: every phantom of the gallery and add_white_noise at N=2048 and N=256, "
+                         "the reference's own functions timed on the host at N=256")
     ap.add_argument("--uneven-count", type=int, default=64)
     ap.add_argument("--batch-iters", type=int, default=300)
     ap.add_argument("--mf-iters", type=int, default=100)
@@ -132,6 +135,8 @@ def main() -> int:
         return bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream)
     if a.mode == "uneven":
         return bench_uneven(a, np, torch, _lib, airms, cf, sg, L, stream)
+    if a.mode == "phantoms":
+        return bench_phantoms(a, np, torch)
 
     g = cf.C2
     dat = sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta())
@@ -508,6 +513,68 @@ def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
            "config": {"workload": "N=256 fan beam 360 x 363 rays, Kaczmarz 20 sweeps, box [0,1]",
                       "nnz": A.nnz, "nonempty_rows": nonempty},
            "rows": rows}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_phantoms(a, np, torch) -> int:
+    """The phantom gallery (phantomgallery.f90) and add_white_noise (system_generation.f90:709-745)
+    on the device: every phantom at N=2048 and N=256 (host-side draws and shape logic included,
+    the runtime's random_number replaced by a seeded stream), the noise for a 2900 x 2048 and a
+    400 x 384 sinogram.  The reference's own functions (oracle/_ref, one host thread, seed 1) are
+    timed at N=256 and on the 400 x 384 sinogram."""
+    from fortran_airtools_amd import phantomgallery as pg, system_generation as sg
+    from oracle.oracle import Reference
+    names = ["shepplogan", "smooth", "threephases", "threephases_smooth", "binary", "fourphases", "mst",
+             "grains", "bubbles"]
+
+    def gpu(name, n):
+        fn = getattr(pg, name)
+        args = {} if name in ("shepplogan", "smooth") else {"draws": pg.Draws(seed=1)}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        img = fn(n, **args)
+        t = time.perf_counter() - t0
+        assert img.shape == (n, n) and np.isfinite(img).all()
+        return t
+
+    ref = Reference() if (Reference.available() and not a.no_cpu) else None
+    rows = []
+    for name in names:
+        gpu(name, 64)                                    # warm-up (kernel loads)
+        t2048, t256 = gpu(name, 2048), gpu(name, 256)
+        t_ref = None
+        if ref is not None:
+            t0 = time.perf_counter()
+            if name == "shepplogan":
+                ref.shepplogan(256)
+            else:
+                ref.phantom(name, 256, 1)
+            t_ref = time.perf_counter() - t0
+        log(f"{name}: device {t2048 * 1e3:.1f} ms at N=2048, {t256 * 1e3:.1f} ms at N=256"
+            + (f"; reference {t_ref * 1e3:.1f} ms at N=256" if t_ref is not None else ""))
+        rows.append({"phantom": name, "gpu_n2048_s": round(t2048, 5), "gpu_n256_s": round(t256, 5),
+                     "reference_n256_s": None if t_ref is None else round(t_ref, 5)})
+    noise = []
+    for m in (2900 * 2048, 400 * 384):
+        rng = np.random.default_rng(2)
+        b, u = rng.random(m), rng.random(m)
+        sg.add_white_noise(b[:1000], 0.01, uniforms=u[:1000])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sg.add_white_noise(b, 0.01, uniforms=u)
+        t = time.perf_counter() - t0
+        t_ref = None
+        if ref is not None and m == 400 * 384:
+            t0 = time.perf_counter()
+            ref.white_noise(b, 0.01, 1)                  # seed 1: the reference's Newton converges
+            t_ref = time.perf_counter() - t0
+        log(f"add_white_noise, {m} samples: device {t * 1e3:.1f} ms"
+            + (f"; reference {t_ref * 1e3:.1f} ms" if t_ref is not None else ""))
+        noise.append({"samples": m, "gpu_s": round(t, 5), "reference_s": None if t_ref is None else round(t_ref, 5)})
+    out = {"metric": METRIC, "mode": "phantoms", "unit": "s", "higher_is_better": False, "dtype": "f64",
+           "config": {"workload": "phantom gallery at N=2048 / N=256 and add_white_noise (seeded draws)"},
+           "rows": rows, "noise": noise}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -57,6 +57,13 @@ SIGNATURES = {
     "fair_matrix_read": [C.c_char_p, _I64, _I32, _P, _P, _P],
     "fair_absorbance": [_P, _I64, _I64, _I64, _P],
     "fair_absorbance_dev": [_P, _I64, _I64, _I64, _P],
+    "fair_phantom_ellipses": [_I32, _P, _P, _I32, _P, _P],
+    "fair_phantom_bumps": [_I32, _P, _P, _I32, _P, _I32, _P],
+    "fair_phantom_nearest": [_I32, _P, _P, _I32, _P, _P],
+    "fair_phantom_last_disk": [_I32, _P, _P, _I32, _P, _P],
+    "fair_phantom_stamp_disks": [_I32, _P, _P, _I32, _P, _D, _P],
+    "fair_phantom_last_band": [_I32, _P, _P, _I32, _P, _P],
+    "fair_normal_newton": [_I64, _P, _P],
     "fair_matvecmul": [_P, _P, _I32, _P, _I32],
     "fair_transvecmul": [_P, _P, _I32, _P, _I32],
     "fair_rowvecmult": [_P, _I32, _P, _I32, _P],

@@ -28,7 +28,7 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-
 # multiply-adds are written out where wanted).
 PER_FILE = {"raytrace.hip": ["-ffp-contract=off"], "sparse.hip": ["-ffp-contract=off"],
             "methods.hip": ["-ffp-contract=off"], "group.hip": ["-ffp-contract=off"],
-            "matfree.hip": ["-ffp-contract=off"]}
+            "matfree.hip": ["-ffp-contract=off"], "phantoms.hip": ["-ffp-contract=off"]}
 
 
 def _compile(src: str) -> str:
@@ -65,7 +65,8 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 def build_fortran(verbose: bool = False) -> str | None:
     """Compile the Fortran drop-in modules (fortran/*.f90, in dependency order) into
     libfair_fortran.a with their .mod files under _build/fmod."""
-    order = ["fair_capi.f90", "sparse_matrices.f90", "system_generation.f90", "airms.f90", "fair_extensions.f90"]
+    order = ["fair_capi.f90", "sparse_matrices.f90", "system_generation.f90", "airms.f90", "phantomgallery.f90",
+             "fair_extensions.f90"]
     srcs = [os.path.join(FORTRAN, f) for f in order if os.path.exists(os.path.join(FORTRAN, f))]
     if not srcs:
         return None

@@ -14,7 +14,9 @@ module fair_capi
               c_fair_transpose, c_fair_multiply_by_scalar, c_fair_add_scalar_mult_row, &
               c_fair_sirt, c_fair_normest, c_fair_gradient_descent, c_fair_art, &
               c_fair_art_uniform_count, c_fair_linspace, c_fair_twin_art, c_fair_twin_art_uniform_bound, &
-              c_fair_art_oracle, c_fair_art_steps
+              c_fair_art_oracle, c_fair_art_steps, c_fair_phantom_ellipses, c_fair_phantom_bumps, &
+              c_fair_phantom_nearest, c_fair_phantom_last_disk, c_fair_phantom_stamp_disks, &
+              c_fair_phantom_last_band, c_fair_normal_newton
 
     ! bind(C) mirror of type(data) without its allocatables (system_generation.f90:6-12)
     type, bind(C) :: fair_geom_c
@@ -38,6 +40,58 @@ module fair_capi
         function c_fair_init(dev) bind(C, name='fair_init') result(s)
             import :: c_int
             integer(c_int), value :: dev
+            integer(c_int) :: s
+        end function
+        ! phantom images (column-major n x n), include/fair_hip.h
+        function c_fair_phantom_ellipses(n, X, Y, ne, ell, img) bind(C, name='fair_phantom_ellipses') result(s)
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value :: n, ne
+            real(c_double), intent(in) :: X(*), Y(*), ell(*)
+            real(c_double) :: img(*)
+            integer(c_int) :: s
+        end function
+        function c_fair_phantom_bumps(n, X, Y, nb, bumps, power, img) bind(C, name='fair_phantom_bumps') result(s)
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value :: n, nb, power
+            real(c_double), intent(in) :: X(*), Y(*), bumps(*)
+            real(c_double) :: img(*)
+            integer(c_int) :: s
+        end function
+        function c_fair_phantom_nearest(n, X, Y, np, pts, img) bind(C, name='fair_phantom_nearest') result(s)
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value :: n, np
+            real(c_double), intent(in) :: X(*), Y(*), pts(*)
+            real(c_double) :: img(*)
+            integer(c_int) :: s
+        end function
+        function c_fair_phantom_last_disk(n, X, Y, nd, disks, img) bind(C, name='fair_phantom_last_disk') result(s)
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value :: n, nd
+            real(c_double), intent(in) :: X(*), Y(*), disks(*)
+            real(c_double) :: img(*)
+            integer(c_int) :: s
+        end function
+        function c_fair_phantom_stamp_disks(n, X, Y, nd, disks, value, img) &
+            bind(C, name='fair_phantom_stamp_disks') result(s)
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value :: n, nd
+            real(c_double), intent(in) :: X(*), Y(*), disks(*)
+            real(c_double), value :: value
+            real(c_double) :: img(*)
+            integer(c_int) :: s
+        end function
+        function c_fair_phantom_last_band(n, X, Y, ne, edges, img) bind(C, name='fair_phantom_last_band') result(s)
+            import :: c_int, c_int32_t, c_double
+            integer(c_int32_t), value :: n, ne
+            real(c_double), intent(in) :: X(*), Y(*), edges(*)
+            real(c_double) :: img(*)
+            integer(c_int) :: s
+        end function
+        function c_fair_normal_newton(m, u, e) bind(C, name='fair_normal_newton') result(s)
+            import :: c_int, c_int64_t, c_double
+            integer(c_int64_t), value :: m
+            real(c_double), intent(in) :: u(*)
+            real(c_double) :: e(*)
             integer(c_int) :: s
         end function
         function c_fair_linspace(a, b, n, x) bind(C, name='fair_linspace') result(s)

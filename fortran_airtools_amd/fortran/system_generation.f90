@@ -252,26 +252,20 @@ contains
         if (verb) write (*, *) "Data collected. Closing file."
     end subroutine read_dump
 
-    ! add_white_noise, system_generation.f90:709-745: per sample a uniform draw u, a Newton
-    ! solve of Phi(x) = u from x = 0 with the reference's step f/df, df = sqrt(2/pi)exp(-x^2/2),
-    ! then e = eta*||b||*e/||e|| and bt = b + e.
+    ! add_white_noise, system_generation.f90:709-745: one uniform draw per sample, in order; the
+    ! Newton solves of Phi(x) = u (from x = 0, step f/df, df = sqrt(2/pi)exp(-x^2/2)) run on the
+    ! device, one thread per sample; then e = eta*||b||*e/||e|| and bt = b + e.
     subroutine add_white_noise(bt, e, b, eta)
         real(kind=8), intent(in) :: b(:), eta
         real(kind=8), dimension(size(b)) :: bt, e
-        real(kind=8) :: u, x, f, df
-        integer :: i, k
+        real(kind=8), allocatable :: u(:)
+        integer :: k
+        allocate (u(size(b)))
         do k = 1, size(b)
-            call random_number(u)
-            x = 0d0
-            do i = 1, 100
-                f = .5d0*(1 + erf(x/sqrt(2d0))) - u
-                df = sqrt(2d0/pi)*exp(-.5d0*x**2)
-                if (abs(f/df) < 1d-14) exit
-                if (i == 100) stop "Newton method did not converge."
-                x = x - f/df
-            end do
-            e(k) = x
+            call random_number(u(k))
         end do
+        call fair_ensure_init()
+        call fair_check(c_fair_normal_newton(int(size(b), c_int64_t), u, e))
         e = eta*norm2(b)*e/norm2(e)
         bt = b + e
     end subroutine add_white_noise

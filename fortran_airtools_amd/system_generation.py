@@ -72,12 +72,27 @@ def paralleltomo(dat: Data) -> CSRMatrix:
     return CSRMatrix(h)
 
 
-def add_white_noise(b, eta: float, g=None, seed: int = 0):
-    """bt = b + e with e = eta*||b||*g/||g|| (system_generation.f90:742-743); g standard normal
-    draws (the reference inverts the normal CDF of RANDOM_NUMBER draws, :725-740).
+def normal_newton(u) -> np.ndarray:
+    """add_white_noise's inverse normal CDF of uniform draws (system_generation.f90:725-739):
+    per draw a Newton solve from 0, on the device."""
+    u = np.ascontiguousarray(u, np.float64).reshape(-1)
+    e = np.empty_like(u)
+    check(lib().fair_normal_newton(int(u.shape[0]), u.ctypes.data_as(C.c_void_p), e.ctypes.data_as(C.c_void_p)))
+    return e
+
+
+def add_white_noise(b, eta: float, g=None, seed: int = 0, uniforms=None):
+    """bt = b + e with e = eta*||b||*g/||g|| (system_generation.f90:742-743).  With `uniforms`
+    (the runtime's random_number draws, one per sample) g is the reference's Newton inversion of
+    them (:725-740, on the device); otherwise g are given or seeded standard normal draws.
     Returns (bt, e)."""
     b = np.asarray(b, np.float64)
-    if g is None:
+    if uniforms is not None:
+        u = np.asarray(uniforms, np.float64).reshape(-1)
+        if u.shape[0] < b.shape[0]:
+            raise FairError(FAIR_ERR_ARG, "one draw per sample is needed")
+        g = normal_newton(u[:b.shape[0]])
+    elif g is None:
         g = np.random.default_rng(seed).standard_normal(b.shape[0])
     e = eta * np.linalg.norm(b) * g / np.linalg.norm(g)
     return b + e, e

@@ -254,6 +254,42 @@ int fair_mf_gd_dev(fair_mf *P, const float *d_b, int32_t maxiter, double rho, do
 int fair_mf_gd(fair_mf *P, const float *b, int32_t maxiter, const float *x0, double rho, double tol,
                double sigma, const fair_opts *opts, float *xout, int32_t *iters);
 
+/* ---- phantom gallery and white noise (per-pixel / per-sample work) ------------------------
+ * phantomgallery.f90 builds its test images pixel by pixel over every shape; the shapes come
+ * from short sequential loops over random_number draws, which the host caller runs as the
+ * reference does and hands over here.  Images are column-major n x n (pixel (i, j) at i + j*n,
+ * i the row), the layout of the reference's x(N,N); X (n) holds the coordinate of each column,
+ * Y (n) that of each row, as meshgrid / intmeshgrid (phantomgallery.f90:17-82) lay them out.
+ * All arrays are host arrays; each call returns after x is copied back. */
+/* shepplogan (phantomgallery.f90:116-164): x = sum of A over the ellipses containing the pixel.
+ * ell: ne x 7 (A, a^2, b^2, x0, y0, cos phi, sin phi). */
+int fair_phantom_ellipses(int32_t n, const double *X, const double *Y, int32_t ne, const double *ell, double *x);
+/* smooth (:166-199), threephases (:201-254), threephases_smooth (:256-328): x = sum over bumps of
+ * w*exp(-(c*P(X-cx))/dx - (c*P(Y-cy))/dy), P(t) = t^2 (power 2) or |t|^3 (power 3).
+ * bumps: nb x 6 (w, cx, cy, dx, dy, c). */
+int fair_phantom_bumps(int32_t n, const double *X, const double *Y, int32_t nb, const double *bumps,
+                       int32_t power, double *x);
+/* grains (:529-597): x = 1-based index of the first seed point nearest in NORM2.  pts: np x 2. */
+int fair_phantom_nearest(int32_t n, const double *X, const double *Y, int32_t np, const double *pts, double *x);
+/* bubbles (:599-642): for each disk in order, pixels with (X-cx)^2+(Y-cy)^2 <= r2 take its value;
+ * x in/out.  disks: nd x 4 (cx, cy, r2, value). */
+int fair_phantom_last_disk(int32_t n, const double *X, const double *Y, int32_t nd, const double *disks,
+                           double *x);
+/* binary (:330-416): pixels with (X-cx)^2+(Y-cy)^2 <= r2 for some disk are set to value; x in/out.
+ * disks: nd x 3 (cx, cy, r2). */
+int fair_phantom_stamp_disks(int32_t n, const double *X, const double *Y, int32_t nd, const double *disks,
+                             double value, double *x);
+/* mst (:439-527, pow = 1): for each tree edge in order, pixels with
+ * |c(X-m1) + s(Y-m2)| + |-s(X-m1) + c(Y-m2)| < r take its value; x in/out.
+ * edges: ne x 6 (m1, m2, c, s, r, value). */
+int fair_phantom_last_band(int32_t n, const double *X, const double *Y, int32_t ne, const double *edges,
+                           double *x);
+/* add_white_noise's per-sample Newton solve of Phi(e) = u (system_generation.f90:725-739), the
+ * reference's step and stopping test; for |e| >= 1.5 the residual is formed with erfc (no
+ * cancellation near 0 and 1).  Fails with "Newton method did not converge." as the reference
+ * stops. */
+int fair_normal_newton(int64_t m, const double *u, double *e);
+
 #ifdef __cplusplus
 }
 #endif

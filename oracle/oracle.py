@@ -299,6 +299,10 @@ class Reference:
         L.ref_rng_seed.argtypes = [i]
         L.ref_rng_draw.argtypes = [i, _F64P]
         L.ref_shepplogan.argtypes = [i, _F64P]
+        L.ref_phantom.argtypes = [i, i, i, i, d, i, i, _F64P]
+        L.ref_white_noise.argtypes = [i, _F64P, d, i, _F64P, _F64P]
+        L.ref_norm2_diff.argtypes = [i, _F64P, _F64P, _F64P]
+        L.ref_powers.argtypes = [i, _F64P, _F64P, _F64P]
         L.ref_linspace.argtypes = [d, d, i, _F64P]
         L.ref_sum.restype = d
         L.ref_sum.argtypes = [i, _F64P]
@@ -468,6 +472,30 @@ class Reference:
         x = np.empty(N * N)
         self.L.ref_shepplogan(N, x)
         return x
+
+    PHANTOM_KINDS = {"smooth": 1, "threephases": 2, "threephases_smooth": 3, "binary": 4, "fourphases": 5,
+                     "mst": 6, "grains": 7, "bubbles": 8}
+
+    def phantom(self, name, N, seed, pin=0, steps=0, v=0.0, gaussian=False):
+        """The reference's phantom after rng_seed(seed), vectorised (column-major)."""
+        x = np.empty(N * N)
+        self.L.ref_phantom(self.PHANTOM_KINDS[name], N, pin, steps, v, int(gaussian), seed, x)
+        return x
+
+    def white_noise(self, b, eta, seed):
+        """add_white_noise after rng_seed(seed): (bt, e)."""
+        b = np.ascontiguousarray(b, np.float64)
+        bt, e = np.empty_like(b), np.empty_like(b)
+        self.L.ref_white_noise(len(b), b, eta, seed, bt, e)
+        return bt, e
+
+    def norm2_diff(self, a, b):
+        """NORM2(a(:, k) - b(:, k)) for 2 x cnt arrays given as cnt x 2."""
+        a = np.ascontiguousarray(a, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        d = np.empty(a.shape[0])
+        self.L.ref_norm2_diff(a.shape[0], a, b, d)
+        return d
 
     def linspace(self, a, b, n):
         x = np.empty(n)

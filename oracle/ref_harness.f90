@@ -424,6 +424,88 @@ contains
         x = vectorise(shepplogan(nn))
     end subroutine ref_shepplogan
 
+    ! A phantom of the reference's gallery (phantomgallery.f90) after ref_rng_seed(seed), as the
+    ! vectorised image.  kind: 1 smooth, 2 threephases, 3 threephases_smooth, 4 binary,
+    ! 5 fourphases, 6 mst, 7 grains, 8 bubbles.  pin / steps / v <= 0 leave that optional argument
+    ! absent; gauss selects randomtype 'gaussian' (grains, bubbles).
+    subroutine ref_phantom(kind, nn, pin, steps, v, gauss, seed, x) bind(C, name='ref_phantom')
+        integer(c_int), value :: kind, nn, pin, steps, gauss, seed
+        real(c_double), value :: v
+        real(c_double), intent(out) :: x(nn*nn)
+        character(8) :: rt
+        real(kind=8) :: vv
+        rt = 'uniform'
+        if (gauss /= 0) rt = 'gaussian'
+        vv = v
+        call ref_rng_seed(seed)
+        select case (kind)
+        case (1)
+            if (pin > 0) then
+                x = vectorise(smooth(nn, pin))
+            else
+                x = vectorise(smooth(nn))
+            end if
+        case (2)
+            if (pin > 0) then
+                x = vectorise(threephases(nn, pin))
+            else
+                x = vectorise(threephases(nn))
+            end if
+        case (3)
+            if (pin > 0 .and. v > 0) then
+                x = vectorise(threephases_smooth(nn, pin, vv))
+            else if (pin > 0) then
+                x = vectorise(threephases_smooth(nn, pin))
+            else
+                x = vectorise(threephases_smooth(nn))
+            end if
+        case (4)
+            if (pin > 0 .and. steps > 0) then
+                x = vectorise(binary(nn, pin, steps))
+            else if (pin > 0) then
+                x = vectorise(binary(nn, pin))
+            else
+                x = vectorise(binary(nn))
+            end if
+        case (5)
+            x = vectorise(fourphases(nn))
+        case (6)
+            if (pin > 0) then
+                x = vectorise(mst(nn, pin))
+            else
+                x = vectorise(mst(nn))
+            end if
+        case (7)
+            if (pin > 0) then
+                x = vectorise(grains(nn, rt, pin))
+            else
+                x = vectorise(grains(nn, rt))
+            end if
+        case (8)
+            if (pin > 0) then
+                x = vectorise(bubbles(nn, rt, pin))
+            else
+                x = vectorise(bubbles(nn, rt))
+            end if
+        case default
+            x = 0d0
+        end select
+    end subroutine ref_phantom
+
+    ! add_white_noise (system_generation.f90:709-745) after ref_rng_seed(seed).
+    subroutine ref_white_noise(m, b, eta, seed, bt, e) bind(C, name='ref_white_noise')
+        integer(c_int), value :: m, seed
+        real(c_double), intent(in) :: b(m)
+        real(c_double), value :: eta
+        real(c_double), intent(out) :: bt(m), e(m)
+        real(kind=8) :: w(m), f(m), ee
+        ee = eta
+        call ref_rng_seed(seed)
+        call add_white_noise(w, f, b, ee)
+        bt = w
+        e = f
+    end subroutine ref_white_noise
+
     subroutine ref_linspace(a, b, nn, x) bind(C, name='ref_linspace')
         real(c_double), value :: a, b
         integer(c_int), value :: nn
@@ -441,6 +523,33 @@ contains
         real(c_double) :: s
         s = sum(v)
     end function ref_sum
+    ! NORM2 of a 2-vector difference and integer powers of a real, in the forms the phantom gallery
+    ! uses them (phantomgallery.f90:555, 584, 612; :230, 283), for pinning the device kernels.
+    subroutine ref_norm2_diff(cnt, a, b, d) bind(C, name='ref_norm2_diff')
+        integer(c_int), value :: cnt
+        real(c_double), intent(in) :: a(2, cnt), b(2, cnt)
+        real(c_double), intent(out) :: d(cnt)
+        real(kind=8) :: xloc(2)
+        integer :: k
+        do k = 1, cnt
+            xloc = a(:, k)
+            d(k) = norm2(xloc - b(:, k))
+        end do
+    end subroutine ref_norm2_diff
+
+    subroutine ref_powers(cnt, v, p2, p3) bind(C, name='ref_powers')
+        integer(c_int), value :: cnt
+        real(c_double), intent(in) :: v(cnt)
+        real(c_double), intent(out) :: p2(cnt), p3(cnt)
+        real(kind=8) :: t
+        integer :: k
+        do k = 1, cnt
+            t = v(k)
+            p2(k) = t**2
+            p3(k) = abs(t)**3
+        end do
+    end subroutine ref_powers
+
     ! A path as the reference's character(30) file-name arguments take it (blank padded).
     function name30(path, plen) result(fname)
         integer(c_int), intent(in) :: plen

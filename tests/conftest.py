@@ -66,3 +66,20 @@ def assert_close_1e12(x, ref, tol=1e-12):
     scale = max(np.max(np.abs(ref[fin])) if fin.any() else 0.0, 1e-300)
     assert r <= tol, f"relative error {r:.3e} > {tol}"
     assert mabs <= tol * scale, f"max-abs error {mabs:.3e} > {tol}*{scale:.3e}"
+
+
+PHANTOM_CHECK = os.path.join(ROOT, "fortran_airtools_amd", "bin", "phantom_check")
+
+
+def runtime_draws(seed, count, tmp_path):
+    """`count` random_number draws of the runtime seeded with put(seed + 7919*i), written by
+    fortran_airtools_amd/bin/phantom_check (the seeding the golden fixtures were made with)."""
+    import subprocess
+    if not os.path.exists(PHANTOM_CHECK):
+        pytest.skip("fortran_airtools_amd/bin/phantom_check is not built")
+    out = tmp_path / f"draws_{seed}_{count}.bin"
+    r = subprocess.run([PHANTOM_CHECK, "draws", str(seed), str(count), str(out)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return np.frombuffer(out.read_bytes(), np.float64)
+

@@ -31,7 +31,12 @@ Outputs (numpy .npz, inputs and expected outputs only):
   sweep_small.npz    (--sweep) basic_sweep on a small fan problem (natural, reversed and
                      row-probability orders, with and without a box) and quicksort of arrays with ties
 
-usage: python tests/golden/make_golden.py [--c2 | --twin | --dump | --data | --artoracle | --sweep]
+  phantoms_small.npz (--phantoms) every phantom of the gallery (default and optional arguments,
+                     both seed-point types) and add_white_noise, each after seeding the runtime
+                     generator with put(seed + 7919*i); the first draws of one seed
+
+usage: python tests/golden/make_golden.py [--c2 | --twin | --dump | --data | --artoracle | --sweep |
+                                           --phantoms]
 """
 from __future__ import annotations
 
@@ -377,6 +382,47 @@ def write_sweep(ref):
     np.savez_compressed(os.path.join(HERE, "sweep_small.npz"), **d)
 
 
+# (name, phantom, N, pin, steps, v, gaussian, seed); pin / steps / v <= 0: argument absent
+PHANTOM_CASES = [
+    ("smooth_48", "smooth", 48, 0, 0, 0.0, False, 0),
+    ("smooth_33_p3", "smooth", 33, 3, 0, 0.0, False, 0),
+    ("threephases_48", "threephases", 48, 0, 0, 0.0, False, 101),
+    ("threephases_40_p30", "threephases", 40, 30, 0, 0.0, False, 102),
+    ("threephases_smooth_48", "threephases_smooth", 48, 0, 0, 0.0, False, 103),
+    ("threephases_smooth_40_p20_v2.5", "threephases_smooth", 40, 20, 0, 2.5, False, 104),
+    ("binary_48", "binary", 48, 0, 0, 0.0, False, 105),
+    ("binary_40_p10_s25", "binary", 40, 10, 25, 0.0, False, 106),
+    ("fourphases_32", "fourphases", 32, 0, 0, 0.0, False, 107),
+    ("mst_48", "mst", 48, 0, 0, 0.0, False, 108),
+    ("mst_40_p25", "mst", 40, 25, 0, 0.0, False, 109),
+    ("grains_48", "grains", 48, 0, 0, 0.0, False, 110),
+    ("grains_48_gaussian", "grains", 48, 0, 0, 0.0, True, 111),
+    ("grains_40_p9", "grains", 40, 9, 0, 0.0, False, 112),
+    ("bubbles_48", "bubbles", 48, 0, 0, 0.0, False, 113),
+    ("bubbles_48_gaussian", "bubbles", 48, 0, 0, 0.0, True, 114),
+    ("bubbles_40_p12", "bubbles", 40, 12, 0, 0.0, False, 115),
+]
+
+
+def write_phantoms(ref):
+    out = {}
+    for name, kind, n, pin, steps, v, gauss, seed in PHANTOM_CASES:
+        out[f"{name}/kind"] = np.array(kind)
+        out[f"{name}/args"] = np.array([n, pin, steps, v, int(gauss), seed], np.float64)
+        out[f"{name}/x"] = ref.phantom(kind, n, seed, pin=pin, steps=steps, v=v, gaussian=gauss)
+    out["shepplogan_48/x"] = ref.shepplogan(48)
+    out["shepplogan_47/x"] = ref.shepplogan(47)
+    rng = np.random.default_rng(21)
+    for k, (m, eta, seed) in enumerate([(500, 0.01, 201), (1237, 0.05, 202)]):
+        b = rng.random(m) * 3.0
+        bt, e = ref.white_noise(b, eta, seed)
+        out[f"noise{k}/b"], out[f"noise{k}/eta"], out[f"noise{k}/seed"] = b, np.array(eta), np.array(seed)
+        out[f"noise{k}/bt"], out[f"noise{k}/e"] = bt, e
+    ref.rng_seed(7)
+    out["draws_seed7"] = ref.rng_draw(64)
+    np.savez_compressed(os.path.join(HERE, "phantoms_small.npz"), **out)
+
+
 def write_c2(ref):
     g = cf.C2
     IA, JA, vA, xt, b = problem(ref, g, cf.grains, 0.005, 2)
@@ -411,6 +457,9 @@ def main():
         return
     if "--sweep" in sys.argv:
         write_sweep(ref)
+        return
+    if "--phantoms" in sys.argv:
+        write_phantoms(ref)
         return
     write_small(ref)
     write_c1(ref)

@@ -3,7 +3,8 @@ project's Fortran modules and run on the GPU, against the same program built wit
 modules, on the same flags.
 
 main.f90 always runs its test problem (main.f90:190-220): paralleltomo with N=256, 400 angles,
-384 rays, the Shepp-Logan phantom and add_white_noise at 5e-3 from the runtime's random_number,
+384 rays, the phantom of -t (Shepp-Logan by default; here also the other phantoms of the drop-in
+phantomgallery module) and add_white_noise at 5e-3 (or -N) from the runtime's random_number,
 then the method of -M for -K iterations, writing sol.txt (and errorgauge.txt with -S).  Both
 binaries draw from the same runtime generator in the same order, so their outputs agree to the
 parity bar (the reference program runs on one thread, see run()).  The binaries are built by
@@ -28,7 +29,9 @@ def run(exe, flags, cwd):
     # its SIRT results change with the thread count; the device gathers in the one-thread order
     env = dict(os.environ, OMP_NUM_THREADS="1")
     r = subprocess.run([exe, *flags], cwd=cwd, capture_output=True, text=True, timeout=300, env=env)
-    assert r.returncode == 0, f"{exe} {flags}: {r.stdout[-2000:]} {r.stderr[-2000:]}"
+    # a Fortran STOP with a message also ends with status 0: require the output file too
+    assert r.returncode == 0 and os.path.exists(os.path.join(cwd, "sol.txt")), \
+        f"{exe} {flags}: status {r.returncode}; {r.stdout[-2000:]} {r.stderr[-2000:]}"
     out = {"sol": np.loadtxt(os.path.join(cwd, "sol.txt"))}
     eg = os.path.join(cwd, "errorgauge.txt")
     if os.path.exists(eg):
@@ -49,6 +52,14 @@ CASES = [
     ("randkaczmarz", ["-Mrand", "-K2", "-L0"]),
     ("graddescent", ["-Mgd", "-K20", "-L0"]),
     ("kaczmarz_oracle_stop", ["-Mkaczmarz", "-K40", "-Soracle", "-R1.5"]),
+    # the other phantoms of -t (the drop-in phantomgallery and add_white_noise on the device)
+    ("smooth_sart", ["-tsmooth", "-Msart", "-K5", "-L0"]),
+    ("3phases_sart", ["-t3phases", "-Msart", "-K5", "-L0"]),
+    ("3phasesmooth_cimmino", ["-t3phasesmooth", "-Mcimmino", "-K5"]),
+    ("binary_sart", ["-tbinary", "-Msart", "-K5", "-L0", "-U1"]),
+    ("mst_sart", ["-tmst", "-Msart", "-K5"]),
+    ("grains_kaczmarz", ["-tgrains", "-Mkaczmarz", "-K1", "-L0"]),
+    ("bubbles_sart", ["-tbubbles", "-Msart", "-K5", "-n0.02"]),
 ]
 
 
