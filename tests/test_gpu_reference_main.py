@@ -85,3 +85,33 @@ def test_reference_program_on_the_gpu(tmp_path, name, flags):
         its = k // 2 if "-Msym" in flags else k
         np.testing.assert_allclose(got["eg"][:its], want["eg"][:its], rtol=1e-12, atol=0)
         assert not got["eg"][its:].any()
+
+
+def test_reference_program_matrix_dump_and_ordering(tmp_path):
+    """main.f90's -s (store_matrix), -r (read_csr_matrix) and -O (read_ordering) paths: both
+    programs store their matrix (the dumps must be byte-identical); the drop-in program reads the
+    reference's dump and matches the reference program that builds the same matrix (the
+    reference's own -r aborts: its read_csr_matrix opens a never-set unit number,
+    system_generation.f90:648); Kaczmarz runs on a custom row ordering file.  Solutions compared
+    to 1e-12."""
+    if not (os.path.exists(REF) and os.path.exists(DROPIN)):
+        pytest.skip("oracle/_ref/main_ref and main_dropin are built only where the reference sources are")
+    a, b = tmp_path / "ref", tmp_path / "gpu"
+    a.mkdir()
+    b.mkdir()
+    rows = np.random.default_rng(4).choice(400 * 384, 3000, replace=False) + 1
+    for d in (a, b):
+        (d / "order.txt").write_text("ordering " + str(len(rows)) + "\n" + "\n".join(str(r) for r in rows) + "\n")
+    stored = ["-smat.bin", "-Msart", "-K5", "-L0"]
+    want, got = run(REF, stored, str(a)), run(DROPIN, stored, str(b))
+    assert rel(got["sol"], want["sol"]) <= 1e-12
+    da, db = (a / "mat.bin").read_bytes(), (b / "mat.bin").read_bytes()
+    assert da == db, "store_matrix dumps differ"
+    (b / "mat.bin").write_bytes(da)                 # the reference's dump
+    del da, db
+    want = run(REF, ["-Mcimmino", "-K5", "-L0"], str(a))
+    got = run(DROPIN, ["-rmat.bin", "-Mcimmino", "-K5", "-L0"], str(b))
+    assert rel(got["sol"], want["sol"]) <= 1e-12
+    ordered = ["-Oorder.txt", "-Mkaczmarz", "-K2", "-L0", "-U1"]
+    want, got = run(REF, ordered, str(a)), run(DROPIN, ordered, str(b))
+    assert rel(got["sol"], want["sol"]) <= 1e-12
