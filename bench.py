@@ -47,7 +47,10 @@ def measured_traffic() -> int | None:
     return None
 
 
-def cpu_baseline(iters: int, method: str = "sart") -> dict | None:
+def cpu_baseline(iters: int, method: str = "sart", geom=None) -> dict | None:
+    """The reference's own method (oracle/_ref, OpenMP, up to 16 host threads) on a fan-beam
+    problem (default N=512, 720 x 725 rays; `geom` a configs.Geometry), timed over a bounded
+    number of iterations."""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -55,6 +58,9 @@ def cpu_baseline(iters: int, method: str = "sart") -> dict | None:
     cores = max(1, min(cores, 16))
     cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--iters", str(iters), "--threads", str(cores),
            "--method", method]
+    if geom is not None:
+        cmd += ["--n", str(geom.n), "--p", str(geom.p), "--nth", str(geom.nth), "--step", str(geom.deg_step),
+                "--r", str(geom.r), "--dw", str(geom.dw), "--sd", str(geom.sd)]
     log("cpu baseline:", " ".join(cmd))
     import resource
 
@@ -277,12 +283,18 @@ def bench_precision(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
                      "fp32_relerr_vs_fp64": err, "fp32_within_1e-5": err <= 1e-5})
         log(f"k={k}: FP64 {t64 * 1e3:.2f} ms, FP32 {t32 * 1e3:.2f} ms, rel. error {err:.2e}")
     k = rows[-1]["iterations"]
+    cpu = None if a.no_cpu else cpu_baseline(a.cpu_iters)
     out = {"metric": METRIC, "mode": "precision", "unit": "s", "dtype": "f32 vs f64",
            "config": {"workload": "N=512 fan beam, 720 x 725 rays, SART, box [0,1]"},
            "rows": rows,
            "time_to_equal_error": {"iterations": k, "fp64_s": rows[-1]["fp64_s"], "fp32_s": rows[-1]["fp32_s"],
                                    "speedup": round(rows[-1]["fp64_s"] / rows[-1]["fp32_s"], 3),
-                                   "fp32_relerr": rows[-1]["fp32_relerr_vs_fp64"]}}
+                                   "fp32_relerr": rows[-1]["fp32_relerr_vs_fp64"],
+                                   "reference_cpu_s": None if cpu is None else round(k / cpu["value"], 3)},
+           "cpu_baseline": cpu}
+    if cpu is not None:
+        log(f"reference on {cpu['cores']} host threads: {k} SART iterations in {k / cpu['value']:.2f} s "
+            f"(from its measured {cpu['value']} it/s)")
     print(json.dumps(out), flush=True)
     return 0
 
@@ -354,6 +366,12 @@ def bench_batch(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
                               "all_within_1e-5": bool((err <= 1e-5).all())},
            "achieved_tflops_fp32": round(flops / t_batch / 1e12, 3),
            "autotuned_rhs_per_wavefront": [kc[0], kc[1]]}
+    cpu = None if a.no_cpu else cpu_baseline(a.cpu_iters, geom=g)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+        out["reference_cpu_s"] = round(k * iters / cpu["value"], 2)
+        log(f"reference on {cpu['cores']} host threads: {cpu['value']} slice-iterations/s, "
+            f"{k * iters / cpu['value']:.1f} s for the whole batch")
     print(json.dumps(out), flush=True)
     return 0
 
