@@ -106,7 +106,8 @@ def main() -> int:
                          "time to equal error at N=512; batch: 256 slices of 256^2 as one batched FP32 run; "
                          "matfree: matrix-free FP32 gradient descent at 2048^2 and at N=512 against the "
                          "FP64 CSR method; kaczmarz: randomized (row-norm sampling) and symmetric "
-                         "Kaczmarz, 20 sweeps at N=256, against the oracle and the reference on the host; "
+                         "Kaczmarz, 20 sweeps at N=256, cyclic Kaczmarz and twin_art, against the oracle and the "
+                         "reference on the host; "
                          "uneven: 64 SART problems from N=128 to N=1024 as one group against one run each; "
                          "phantoms: every phantom of the gallery and add_white_noise at N=2048 and N=256, "
                          "the reference's own functions timed on the host at N=256")
@@ -484,9 +485,10 @@ def bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
 def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     """Kaczmarz at N=256 (fan beam 360 x 363 rays, three-phase phantom, 1 % noise, seed 5), box
     [0,1], 20 sweeps: randomized Kaczmarz with row-norm sampling (the reference's random_number
-    draws supplied from a seeded generator) and symmetric Kaczmarz (K=20: 10 forward + reverse
-    pairs).  Each GPU result is checked against the oracle run on the same draws; the reference's
-    own art (one host thread, as it is serial) is timed on the same problem."""
+    draws supplied from a seeded generator), symmetric Kaczmarz (K=20: 10 forward + reverse
+    pairs) and cyclic Kaczmarz (K=20), and twin_art with symmetric Kaczmarz and the errorgauge rule.
+    Each GPU result is checked against the oracle run on the same draws; the reference's own art
+    and twin_art (one host thread, as they are serial) are timed on the same problem."""
     g = cf.C3
     dat = sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta())
     A = sg.fanlineartomo(dat)
@@ -499,12 +501,18 @@ def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     build_oracle()
     orc = Oracle()
     rows = []
-    for method, rowprob in (("randkaczmarz", True), ("symkaczmarz", False)):
+    for method, rowprob in (("randkaczmarz", True), ("symkaczmarz", False), ("kaczmarz", False)):
         o.rowprobability = rowprob
         u = None
         if method == "randkaczmarz":
             u = np.random.default_rng(5).random(airms.art_uniform_count(method, A, [20], options=o))
-        airms.art(method, A, b, [1 if method == "randkaczmarz" else 2], options=o, uniforms=u)   # warm-up
+        # warm-up: one iteration; for the cyclic orders it also makes the level schedule the timed
+        # run reuses (cached in the matrix), so its time is reported as first_call_s
+        k_warm = 2 if method == "symkaczmarz" else 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        airms.art(method, A, b, [k_warm], options=o, uniforms=u)
+        t_first = time.perf_counter() - t0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         X = airms.art(method, A, b, [20], options=o, uniforms=u)
@@ -526,7 +534,30 @@ def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
                      "gpu_s": round(t_gpu, 4), "rows_per_s": round(steps / t_gpu, 1),
                      "nnz_per_s": round(20 * A.nnz / t_gpu, 1), "relerr_vs_oracle": err,
                      "within_1e-12": err <= 1e-12,
+                     "first_call_s": round(t_first, 4), "first_call_sweeps": k_warm,
                      "reference_host_s": None if t_ref is None else round(t_ref, 4)})
+    # twin_art, symmetric Kaczmarz with the errorgauge rule: two chains per iteration, up to 20
+    # sweeps each (the rule may stop earlier; final_it says where)
+    o.rowprobability = False
+    airms.twin_art("symkaczmarz", "errorgauge", A, b, [2], options=o)          # warm-up (schedules)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xt_, fit, _, _, _ = airms.twin_art("symkaczmarz", "errorgauge", A, b, [20], options=o)
+    t_gpu = time.perf_counter() - t0
+    xo_, fio, _, _, _, _ = orc.twin_art("symkaczmarz", "errorgauge", IA, JA, vA, A.n, b, [20], lb=0.0, ub=1.0)
+    err = float(np.linalg.norm(xt_ - xo_) / np.linalg.norm(xo_))
+    t_ref = None
+    if Reference.available() and not a.no_cpu:
+        ref = Reference()
+        ref.set(0, IA, JA, vA, A.n)
+        t0 = time.perf_counter()
+        ref.twin_art(0, "symkaczmarz", "errorgauge", b, [20], lb=0.0, ub=1.0)
+        t_ref = time.perf_counter() - t0
+    log(f"twin_art symkaczmarz errorgauge: GPU {t_gpu:.3f} s (final_it {fit}, oracle {fio}), rel. error to "
+        f"oracle {err:.2e}" + (f"; reference on one host thread {t_ref:.3f} s" if t_ref else ""))
+    rows.append({"method": "twin_art symkaczmarz errorgauge", "kin": 20, "final_it": fit, "final_it_oracle": fio,
+                 "gpu_s": round(t_gpu, 4), "relerr_vs_oracle": err, "within_1e-12": err <= 1e-12,
+                 "reference_host_s": None if t_ref is None else round(t_ref, 4)})
     out = {"metric": METRIC, "mode": "kaczmarz", "unit": "rows/s", "dtype": "f64",
            "config": {"workload": "N=256 fan beam 360 x 363 rays, Kaczmarz 20 sweeps, box [0,1]",
                       "nnz": A.nnz, "nonempty_rows": nonempty},

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -110,6 +111,36 @@ struct Blocked {
     DBuf<uint16_t> loc;         // per entry: LDS slot of its column (parallel to JA)
 };
 
+// Level schedule of one Kaczmarz row sequence (methods.hip): step s gets level
+// 1 + max(level of the last earlier step touching one of its pixels), so the steps of one level
+// touch disjoint pixels and every pixel sees its updates in sequence order.  rows holds the
+// steps sorted by (level, position), off the level offsets (nlev + 1).  Depends on the sparsity
+// pattern only, which a Csr never changes after it is built.
+// Per-call arguments of a captured level graph, read by its kernels from device memory.
+struct ArtParams {
+    double *x;
+    const double *b, *nrA;
+    double relax, lb, ub;
+    int lwrb, upb;
+};
+
+struct LevelSched {
+    std::vector<int32_t> seq;   // the one-based row sequence it was made for
+    std::vector<int32_t> hoff;  // host copy of off
+    DBuf<int32_t> rows, off;
+    int32_t nlev = 0;
+    // one kernel per level, captured once into a graph; its arguments come from params
+    hipGraphExec_t graph = nullptr;
+    DBuf<ArtParams> params;
+    ArtParams hparams{};
+    LevelSched() = default;
+    LevelSched(const LevelSched &) = delete;
+    LevelSched &operator=(const LevelSched &) = delete;
+    ~LevelSched() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
+};
+
 struct Csr {
     int32_t m = 0, n = 0;
     int64_t nnz = 0;
@@ -122,6 +153,7 @@ struct Csr {
     int32_t n_rows_empty = 0;
     int32_t max_row = 0;     // longest row
     Blocked blk;             // LDS-staged layout (built for ray-traced matrices)
+    std::vector<std::unique_ptr<LevelSched>> lev;   // Kaczmarz level schedules, oldest first
 };
 
 }  // namespace fair

@@ -371,6 +371,338 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, con
     }
 }
 
+// ---- level-parallel sweep ----------------------------------------------------------------
+// The same row steps as k_art_sweep, in an order that keeps every pixel's update sequence: the
+// steps of one level (LevelSched) touch disjoint pixels, so the LEV_WAVES wavefronts of one
+// workgroup take them side by side (wavefront w: steps off[l] + w, + LEV_WAVES, ...) and a
+// workgroup barrier separates the levels.  One workgroup per chain keeps each chain's stores and
+// loads on one CU, where the barrier orders them.  Each row is computed exactly as k_art_sweep
+// computes it (entries of lane q: beg + q + 64 t in increasing t, the same DPP reduction and
+// update), so the iterate equals k_art_sweep's bit for bit.  A wavefront loads the entries of its
+// next step (possibly a level ahead) while it processes the current one.
+constexpr int LEV_WAVES = 16;
+constexpr int LEV_THREADS = LEV_WAVES * WAVE;
+
+struct LevelChains {
+    const int32_t *rows[2];
+    const int32_t *off[2];
+    int32_t nlev[2];
+    double *x[2];
+};
+
+// Steps the wavefront's (level, step) to its next step: the next one of this wavefront in level
+// l, else its slot in the next level wide enough to have one; l == nlev when none is left.
+__device__ __forceinline__ void lev_advance(const int32_t *__restrict__ off, int32_t nlev, int32_t wv,
+                                            int32_t &l, int32_t &s) {
+    s += LEV_WAVES;
+    while (l < nlev && s >= off[l + 1]) {
+        ++l;
+        s = off[l] + wv;
+    }
+}
+
+__device__ __forceinline__ ArtRow lev_row(const int32_t *rows, bool ok, int32_t s, const int32_t *IA,
+                                          const double *b, const double *nrA) {
+    ArtRow d;
+    d.r = rows[ok ? s : 0] - 1;
+    d.beg = IA[d.r];
+    d.end = ok ? IA[d.r + 1] : d.beg;
+    d.b = b[d.r];
+    d.w = nrA[d.r];
+    return d;
+}
+
+__global__ void __launch_bounds__(LEV_THREADS) k_art_levels(LevelChains ch, const int32_t *__restrict__ IA,
+                                                            const int32_t *__restrict__ JA,
+                                                            const double *__restrict__ vA,
+                                                            const double *__restrict__ b,
+                                                            const double *__restrict__ nrA,
+                                                            int32_t ncols, double relax, int lwrb,
+                                                            double lb, int upb, double ub) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int32_t wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int32_t *__restrict__ rows = ch.rows[blockIdx.x];
+    const int32_t *__restrict__ off = ch.off[blockIdx.x];
+    const int32_t nlev = ch.nlev[blockIdx.x];
+    double *x = ch.x[blockIdx.x];
+    if (nlev <= 0) return;                 // the whole workgroup leaves together
+    int32_t lc = 0, sc = wv - LEV_WAVES;
+    lev_advance(off, nlev, wv, lc, sc);
+    int32_t ln = lc, sn = sc;
+    lev_advance(off, nlev, wv, ln, sn);
+    ArtRow cur = lev_row(rows, lc < nlev, sc, IA, b, nrA);
+    ArtRow nxt = lev_row(rows, ln < nlev, sn, IA, b, nrA);
+    int32_t jc[ART_R];
+    double vc[ART_R];
+#pragma unroll
+    for (int u = 0; u < ART_R; ++u) {
+        const int32_t k = cur.beg + lane + WAVE * u;
+        const int32_t kk = k < cur.end ? k : 0;
+        const int32_t jv = JA[kk];
+        const double vv = vA[kk];
+        jc[u] = k < cur.end ? jv : -1;
+        vc[u] = k < cur.end ? vv : 0.0;
+    }
+    for (int32_t l = 0; l < nlev; ++l) {
+        while (lc == l) {
+            double xv[ART_R];
+#pragma unroll
+            for (int u = 0; u < ART_R; ++u) {
+                const bool ok = (uint32_t)jc[u] < (uint32_t)ncols;
+                const double xx = x[ok ? jc[u] : 0];
+                xv[u] = ok ? xx : 0.0;
+            }
+            int32_t jn[ART_R];
+            double vn[ART_R];
+#pragma unroll
+            for (int u = 0; u < ART_R; ++u) {
+                const int32_t k = nxt.beg + lane + WAVE * u;
+                const int32_t kk = k < nxt.end ? k : 0;
+                const int32_t jv = JA[kk];
+                const double vv = vA[kk];
+                jn[u] = k < nxt.end ? jv : -1;
+                vn[u] = k < nxt.end ? vv : 0.0;
+            }
+            int32_t l2 = ln, s2 = sn;
+            lev_advance(off, nlev, wv, l2, s2);
+            const ArtRow nn = lev_row(rows, l2 < nlev, s2, IA, b, nrA);
+            double dot = 0.0;
+#pragma unroll
+            for (int u = 0; u < ART_R; ++u) dot += xv[u] * vc[u];
+            for (int32_t k = cur.beg + lane + WAVE * ART_R; k < cur.end; k += WAVE) {
+                const int j = JA[k];
+                if ((uint32_t)j < (uint32_t)ncols) dot += x[j] * vA[k];
+            }
+            dot = wave_sum_dpp(dot);
+            const double coeff = (relax * (cur.b - dot)) / cur.w;
+#pragma unroll
+            for (int u = 0; u < ART_R; ++u) {
+                if ((uint32_t)jc[u] < (uint32_t)ncols) {
+                    double v = xv[u] + coeff * vc[u];
+                    if (lwrb) v = fmax_ref(v, lb);
+                    if (upb) v = fmin_ref(v, ub);
+                    x[jc[u]] = v;
+                }
+            }
+            for (int32_t k = cur.beg + lane + WAVE * ART_R; k < cur.end; k += WAVE) {
+                const int j = JA[k];
+                if ((uint32_t)j >= (uint32_t)ncols) continue;
+                double v = x[j] + coeff * vA[k];
+                if (lwrb) v = fmax_ref(v, lb);
+                if (upb) v = fmin_ref(v, ub);
+                x[j] = v;
+            }
+            cur = nxt;
+            lc = ln;
+            nxt = nn;
+            ln = l2;
+            sn = s2;
+#pragma unroll
+            for (int u = 0; u < ART_R; ++u) {
+                jc[u] = jn[u];
+                vc[u] = vn[u];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One level of a schedule as its own launch: one wavefront per step (steps s0..s1 of the level,
+// which touch disjoint pixels), spread over the whole device; the launches of consecutive levels
+// are ordered by the stream (a captured graph), which also makes each level's stores visible to
+// the next.  Each row is computed as k_art_sweep computes it.
+constexpr int LEVK_THREADS = WAVE;
+
+__global__ void __launch_bounds__(LEVK_THREADS) k_art_level(const ArtParams *__restrict__ P,
+                                                            const int32_t *__restrict__ rows, int32_t s0,
+                                                            int32_t s1, const int32_t *__restrict__ IA,
+                                                            const int32_t *__restrict__ JA,
+                                                            const double *__restrict__ vA, int32_t ncols) {
+    const int lane = threadIdx.x;
+    const int32_t s = s0 + (int32_t)blockIdx.x;
+    if (s >= s1) return;
+    double *x = P->x;
+    const double relax = P->relax, lb = P->lb, ub = P->ub;
+    const int lwrb = P->lwrb, upb = P->upb;
+    const int32_t r = rows[s] - 1;
+    const int32_t beg = IA[r], end = IA[r + 1];
+    const double br = P->b[r], wr = P->nrA[r];
+    int32_t jc[ART_R];
+    double vc[ART_R], xv[ART_R];
+#pragma unroll
+    for (int u = 0; u < ART_R; ++u) {
+        const int32_t k = beg + lane + WAVE * u;
+        const int32_t kk = k < end ? k : 0;
+        const int32_t jv = JA[kk];
+        const double vv = vA[kk];
+        jc[u] = k < end ? jv : -1;
+        vc[u] = k < end ? vv : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < ART_R; ++u) {
+        const bool ok = (uint32_t)jc[u] < (uint32_t)ncols;
+        const double xx = x[ok ? jc[u] : 0];
+        xv[u] = ok ? xx : 0.0;
+    }
+    double dot = 0.0;
+#pragma unroll
+    for (int u = 0; u < ART_R; ++u) dot += xv[u] * vc[u];
+    for (int32_t k = beg + lane + WAVE * ART_R; k < end; k += WAVE) {
+        const int j = JA[k];
+        if ((uint32_t)j < (uint32_t)ncols) dot += x[j] * vA[k];
+    }
+    dot = wave_sum_dpp(dot);
+    const double coeff = (relax * (br - dot)) / wr;
+#pragma unroll
+    for (int u = 0; u < ART_R; ++u) {
+        if ((uint32_t)jc[u] < (uint32_t)ncols) {
+            double v = xv[u] + coeff * vc[u];
+            if (lwrb) v = fmax_ref(v, lb);
+            if (upb) v = fmin_ref(v, ub);
+            x[jc[u]] = v;
+        }
+    }
+    for (int32_t k = beg + lane + WAVE * ART_R; k < end; k += WAVE) {
+        const int j = JA[k];
+        if ((uint32_t)j >= (uint32_t)ncols) continue;
+        double v = x[j] + coeff * vA[k];
+        if (lwrb) v = fmax_ref(v, lb);
+        if (upb) v = fmin_ref(v, ub);
+        x[j] = v;
+    }
+}
+
+hipStream_t g_level_capture = nullptr;
+
+int env_flag(const char *name, int dflt) {
+    const char *v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
+// The level schedule of a row sequence (one-based rows), made on the host from the pattern and
+// cached in the matrix; nullptr when it is switched off (FAIR_ART_LEVELS=0) or the sequence has
+// fewer than FAIR_ART_LEVEL_WIDTH (4) steps per level on average, where one wavefront walking the
+// sequence is as fast.
+const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
+    if (S <= 0 || S > INT32_MAX || A.n <= 0 || !env_flag("FAIR_ART_LEVELS", 1)) return nullptr;
+    const int64_t width = std::max(1, env_flag("FAIR_ART_LEVEL_WIDTH", 4));
+    for (auto &c : A.lev)
+        if ((int64_t)c->seq.size() == S && std::equal(c->seq.begin(), c->seq.end(), seq))
+            return S >= width * c->nlev ? c.get() : nullptr;
+    auto c = std::make_unique<LevelSched>();
+    c->seq.assign(seq, seq + S);
+    std::vector<int32_t> IA((size_t)A.m + 1), JA((size_t)std::max<int64_t>(A.nnz, 1));
+    A.IA.download(IA.data(), (size_t)A.m + 1);
+    A.JA.download(JA.data(), (size_t)A.nnz);
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    std::vector<int32_t> last((size_t)A.n, 0), lev((size_t)S);
+    int32_t L = 0;
+    for (int64_t s = 0; s < S; ++s) {
+        const int32_t r = seq[s] - 1;
+        int32_t mx = 0;
+        for (int32_t k = IA[r]; k < IA[r + 1]; ++k) {
+            const int32_t j = JA[k];
+            if ((uint32_t)j < (uint32_t)A.n && last[j] > mx) mx = last[j];
+        }
+        ++mx;
+        for (int32_t k = IA[r]; k < IA[r + 1]; ++k) {
+            const int32_t j = JA[k];
+            if ((uint32_t)j < (uint32_t)A.n) last[j] = mx;
+        }
+        lev[s] = mx;
+        L = std::max(L, mx);
+    }
+    c->nlev = L;
+    std::vector<int32_t> off((size_t)L + 1, 0), ordered((size_t)S);
+    for (int64_t s = 0; s < S; ++s) ++off[lev[s]];
+    for (int32_t l = 0; l < L; ++l) off[l + 1] += off[l];
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (int64_t s = 0; s < S; ++s) ordered[fill[lev[s] - 1]++] = seq[s];
+    c->rows.alloc((size_t)S);
+    c->off.alloc((size_t)L + 1);
+    c->rows.upload(ordered.data(), (size_t)S);
+    c->off.upload(off.data(), (size_t)L + 1);
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    c->hoff = std::move(off);
+    if (A.lev.size() >= 6) A.lev.erase(A.lev.begin());
+    A.lev.push_back(std::move(c));
+    return S >= width * L ? A.lev.back().get() : nullptr;
+}
+
+// The schedule as one launch per level: captured into a graph on first use, replayed on the
+// library stream after the call's arguments are copied to params.  At N = 256 (about 44 steps per
+// level) a level takes 3.8 us this way against 13.3 us in one workgroup, whose single CU issues
+// every x gather and store of the level.
+void graph_art_levels(const Csr &A, LevelSched *s, double *x, const double *b, const double *nrA, double relax,
+                      const Box &box, hipStream_t st) {
+    if (!s->graph) {
+        if (!g_level_capture) FAIR_HIP(hipStreamCreateWithFlags(&g_level_capture, hipStreamNonBlocking));
+        s->params.alloc(1);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        hipGraph_t g;
+        FAIR_HIP(hipStreamBeginCapture(g_level_capture, hipStreamCaptureModeThreadLocal));
+        for (int32_t l = 0; l < s->nlev; ++l) {
+            const int32_t w = s->hoff[l + 1] - s->hoff[l];
+            hipLaunchKernelGGL(k_art_level, dim3((unsigned)w), dim3(LEVK_THREADS), 0, g_level_capture, s->params.p,
+                               s->rows.p, s->hoff[l], s->hoff[l + 1], A.IA.p, A.JA.p, A.vA.p, A.n);
+        }
+        hipError_t le = hipGetLastError();
+        FAIR_HIP(hipStreamEndCapture(g_level_capture, &g));
+        FAIR_HIP(le);
+        hipGraphExec_t ex;
+        hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FAIR_HIP(ie);
+        s->graph = ex;
+    }
+    s->hparams = ArtParams{x, b, nrA, relax, box.lb, box.ub, box.lwrb, box.upb};
+    FAIR_HIP(hipMemcpyAsync(s->params.p, &s->hparams, sizeof(ArtParams), hipMemcpyHostToDevice, st));
+    FAIR_HIP(hipGraphLaunch(s->graph, st));
+    launch_check("art level graph");
+}
+
+hipStream_t g_level_side = nullptr;
+hipEvent_t g_level_ev[2] = {nullptr, nullptr};
+
+// One level-parallel sweep per chain (s1 / x1 may be null): one launch per level per chain, or
+// with FAIR_ART_LEVEL_MODE=workgroup one workgroup per chain.
+void launch_art_levels(const Csr &A, const LevelSched *s0, double *x0, const LevelSched *s1, double *x1,
+                       const double *b, const double *nrA, double relax, const Box &box) {
+    const char *mode = getenv("FAIR_ART_LEVEL_MODE");
+    if (!(mode && !strcmp(mode, "workgroup"))) {
+        if (env_flag("FAIR_DEBUG", 0))
+            fprintf(stderr, "[fair] level sweep (graph): %zu steps in %d levels%s\n", s0->seq.size(), s0->nlev,
+                    s1 ? " (two chains)" : "");
+        if (!s1) {
+            graph_art_levels(A, const_cast<LevelSched *>(s0), x0, b, nrA, relax, box, stream());
+            return;
+        }
+        // two independent chains (twin_art): the second replays on a side stream alongside the
+        // first, both ordered after the caller's work and before whatever follows on the stream
+        if (!g_level_side) {
+            FAIR_HIP(hipStreamCreateWithFlags(&g_level_side, hipStreamNonBlocking));
+            for (auto &e : g_level_ev) FAIR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        FAIR_HIP(hipEventRecord(g_level_ev[0], stream()));
+        FAIR_HIP(hipStreamWaitEvent(g_level_side, g_level_ev[0], 0));
+        graph_art_levels(A, const_cast<LevelSched *>(s1), x1, b, nrA, relax, box, g_level_side);
+        graph_art_levels(A, const_cast<LevelSched *>(s0), x0, b, nrA, relax, box, stream());
+        FAIR_HIP(hipEventRecord(g_level_ev[1], g_level_side));
+        FAIR_HIP(hipStreamWaitEvent(stream(), g_level_ev[1], 0));
+        return;
+    }
+    LevelChains lc{{s0->rows.p, s1 ? s1->rows.p : nullptr},
+                   {s0->off.p, s1 ? s1->off.p : nullptr},
+                   {s0->nlev, s1 ? s1->nlev : 0},
+                   {x0, x1}};
+    if (env_flag("FAIR_DEBUG", 0))
+        fprintf(stderr, "[fair] level sweep: %zu steps in %d levels%s\n", s0->seq.size(), s0->nlev,
+                s1 ? " (two chains)" : "");
+    hipLaunchKernelGGL(k_art_levels, dim3(s1 ? 2 : 1), dim3(LEV_THREADS), 0, stream(), lc, A.IA.p, A.JA.p,
+                       A.vA.p, b, nrA, A.n, relax, box.lwrb, box.lb, box.upb, box.ub);
+    launch_check("art level sweep");
+}
+
 __global__ void k_art_weights(double *nrA, int32_t m, double add) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < m) nrA[i] = nrA[i] + add;
@@ -704,10 +1036,15 @@ extern "C" int fair_art_steps(fair_csr *h, const int32_t *rows, int64_t nsteps, 
         dx.upload(x, n);
         const Box box = make_box(o);
         const double relax = o ? o->relaxpar : 1.0;
-        hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
-                           SweepChains{{dr.p, nullptr}, {nsteps, 0}, {dx.p, nullptr}}, h->A.IA.p, h->A.JA.p,
-                           h->A.vA.p, db.p, dw.p, n, relax, box.lwrb, box.lb, box.upb, box.ub);
-        launch_check("art sweep");
+        // a caller repeating one sequence (cyclic or symmetric sweeps) reuses its level schedule
+        if (const LevelSched *lev = art_levels(h->A, rows, nsteps)) {
+            launch_art_levels(h->A, lev, dx.p, nullptr, nullptr, db.p, dw.p, relax, box);
+        } else {
+            hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
+                               SweepChains{{dr.p, nullptr}, {nsteps, 0}, {dx.p, nullptr}}, h->A.IA.p, h->A.JA.p,
+                               h->A.vA.p, db.p, dw.p, n, relax, box.lwrb, box.lb, box.upb, box.ub);
+            launch_check("art sweep");
+        }
         dx.download(x, n);
         FAIR_HIP(hipStreamSynchronize(stream()));
     });
@@ -792,6 +1129,8 @@ extern "C" int fair_twin_art(fair_csr *h, int32_t method, int32_t stoprule, cons
             q += 2 * (int64_t)P.nord;
         };
         const unsigned nb = grid_for(n, 256);
+        const LevelSched *lx = nullptr, *ly = nullptr;
+        bool planned = false;
         // both chains' sweeps of one iteration, draws taken in the reference's call order
         auto sweep_both = [&] {
             q = sweep_rows(P, ord, false, uniforms, nu, q, m, sx.data());
@@ -799,6 +1138,15 @@ extern "C" int fair_twin_art(fair_csr *h, int32_t method, int32_t stoprule, cons
             q = sweep_rows(P, ord, true, uniforms, nu, q, m, sy.data());
             if (sym) q = sweep_rows(P, ord, false, uniforms, nu, q, m, sy.data() + P.nord);
             if (per_chain <= 0) return;
+            if (!planned && !P.isrand) {   // without draws both chains repeat their sequences
+                lx = art_levels(h->A, sx.data(), per_chain);
+                ly = lx ? art_levels(h->A, sy.data(), per_chain) : nullptr;
+            }
+            planned = true;
+            if (lx && ly) {
+                launch_art_levels(h->A, lx, x.p, ly, y.p, db.p, d_nrA.p, relax, box);
+                return;
+            }
             dsx.upload(sx.data(), per_chain);
             dsy.upload(sy.data(), per_chain);
             hipLaunchKernelGGL(k_art_sweep, dim3(2), dim3(SWEEP_THREADS), 0, stream(),
@@ -941,6 +1289,7 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
     size_t rec = 0;
     if (exit_iter) *exit_iter = 0;
     const unsigned nb = grid_for(n, 256);
+    const LevelSched *lev = nullptr;
     for (int iter = 1; iter <= P.maxiter; ++iter) {
         FAIR_CHECK(q + per_iter <= nu || !P.isrand, FAIR_ERR_ARG, "too few uniforms: need " + std::to_string(need));
         if (P.isrand) {                               // :320
@@ -960,7 +1309,11 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
         if (P.method == FAIR_SYMKACZMARZ)             // :326-328
             for (int ii = 0; ii < P.nord; ++ii) seq[P.nord + ii] = ord[P.nord - 1 - ii];
         if (xref && n > 0) hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, stream(), dx.p, dxp.p, (int64_t)n);
-        if (steps_per_iter > 0) {
+        // without draws every iteration sweeps the same sequence: one level schedule serves all
+        if (iter == 1 && !P.isrand) lev = art_levels(h->A, seq.data(), steps_per_iter);
+        if (lev) {
+            launch_art_levels(h->A, lev, dx.p, nullptr, nullptr, db.p, d_nrA.p, relax, box);
+        } else if (steps_per_iter > 0) {
             dseq.upload(seq.data(), steps_per_iter);
             hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
                                SweepChains{{dseq.p, nullptr}, {(int64_t)steps_per_iter, 0}, {dx.p, nullptr}},

@@ -202,7 +202,10 @@ int fair_twin_art(fair_csr *A, int32_t method, int32_t stoprule, const double *b
 int fair_twin_art_uniform_bound(fair_csr *A, int32_t method, int32_t nk, const int32_t *Kin,
                                 const int32_t *order, int32_t nord, const fair_opts *opts, int64_t *count);
 /* basic_sweep, AIRMs.f90:813-858: the row steps over an explicit one-based row sequence (the
- * caller makes the row choice: ord, or the row-probability draws), as one persistent launch.
+ * caller makes the row choice: ord, or the row-probability draws).  A sequence with at least 4
+ * steps per level of its level schedule (steps of a level touch disjoint pixels) runs level by
+ * level; the schedule is cached in the matrix, so repeating one sequence plans it once.  Other
+ * sequences run as one persistent launch.  The iterate is the same bit for bit either way.
  * nrA: the rows' (damped squared) norms (m); x (n) updated in place.  Host arrays; rows outside
  * 1..m are rejected. */
 int fair_art_steps(fair_csr *A, const int32_t *rows, int64_t nsteps, const double *b, const double *nrA,
