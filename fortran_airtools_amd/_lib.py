@@ -93,6 +93,7 @@ SIGNATURES = {
     "fair_art_uniform_count": [_P, _I32, _I32, _P, _P, _I32, C.POINTER(Opts), C.POINTER(_I64)],
     "fair_art_steps": [_P, _P, _I64, _P, _P, C.POINTER(Opts), _P],
     "fair_art_steps_dev": [_P, _P, _I64, _P, _P, C.POINTER(Opts), _P],
+    "fair_level_schedule": [_I32, _I32, _P, _P, _P, _I64, _P, _P, C.POINTER(C.c_int32)],
     "fair_twin_art": [_P, _I32, _I32, _P, _I32, _P, _P, _P, _P, _I32, _I32, C.POINTER(Opts), _P, _I64, _P,
                       C.POINTER(_I32), _P, _P, _P, C.POINTER(_I64)],
     "fair_twin_art_uniform_bound": [_P, _I32, _I32, _P, _P, _I32, C.POINTER(Opts), C.POINTER(_I64)],

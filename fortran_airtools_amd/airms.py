@@ -231,6 +231,27 @@ def basic_sweep(x, A: CSRMatrix, b, ord_, nord: int, isrand: bool, rowprob: bool
     return xs
 
 
+def level_schedule(m: int, n: int, IA, JA, seq):
+    """The level schedule the Kaczmarz sweeps run a row sequence in (host work, no device call):
+    returns (rows, off), the sequence's one-based rows sorted by (level, position) and the level
+    offsets.  Steps of one level touch disjoint columns; every column sees its steps in sequence
+    order, so the sweep's result does not change.  IA zero-based offsets, JA one-based columns."""
+    IA = np.ascontiguousarray(IA, np.int32)
+    JA = np.ascontiguousarray(JA, np.int32)
+    seq = np.ascontiguousarray(seq, np.int32)
+    if IA.shape[0] != m + 1:
+        raise FairError(FAIR_ERR_ARG, "IA must hold m + 1 offsets")
+    if JA.shape[0] < IA[m]:
+        raise FairError(FAIR_ERR_ARG, "JA holds fewer than IA(m) entries")
+    rows = np.zeros(max(seq.shape[0], 1), np.int32)
+    off = np.zeros(seq.shape[0] + 1, np.int32)
+    nlev = C.c_int32()
+    # host work only: the library is loaded but no device is initialised
+    check(_lib.load().fair_level_schedule(m, n, _ptr(IA), _ptr(JA), _ptr(seq), seq.shape[0], _ptr(rows),
+                                          _ptr(off), C.byref(nlev)))
+    return rows[: seq.shape[0]], off[: nlev.value + 1]
+
+
 def art_steps_dev(A: CSRMatrix, d_rows: int, nsteps: int, d_b: int, d_nrA: int, omg: float, d_x: int,
                   ub: float | None = None, lb: float | None = None) -> None:
     """The row steps of basic_sweep (AIRMs.f90:813-858) with device addresses: int32 one-based rows

@@ -579,6 +579,38 @@ int env_flag(const char *name, int dflt) {
     return v && *v ? atoi(v) : dflt;
 }
 
+// Levels of a row sequence (one-based rows) over a CSR pattern whose column c is JA[k] - base
+// (columns outside 0..n-1 are ignored, as the sweeps ignore them): step s gets level
+// 1 + max(level of the last earlier step touching one of its pixels).  Writes the steps sorted
+// by (level, position) to ordered (S) and the level offsets to off (levels + 1 <= S + 1 entries);
+// returns the number of levels.
+int32_t levelize(int32_t n, const int32_t *IA, const int32_t *JA, int32_t base, const int32_t *seq, int64_t S,
+                 int32_t *ordered, int32_t *off) {
+    std::vector<int32_t> last((size_t)std::max(n, 1), 0), lev((size_t)S);
+    int32_t L = 0;
+    for (int64_t s = 0; s < S; ++s) {
+        const int32_t r = seq[s] - 1;
+        int32_t mx = 0;
+        for (int32_t k = IA[r]; k < IA[r + 1]; ++k) {
+            const int32_t j = JA[k] - base;
+            if ((uint32_t)j < (uint32_t)n && last[j] > mx) mx = last[j];
+        }
+        ++mx;
+        for (int32_t k = IA[r]; k < IA[r + 1]; ++k) {
+            const int32_t j = JA[k] - base;
+            if ((uint32_t)j < (uint32_t)n) last[j] = mx;
+        }
+        lev[s] = mx;
+        L = std::max(L, mx);
+    }
+    std::fill(off, off + L + 1, 0);
+    for (int64_t s = 0; s < S; ++s) ++off[lev[s]];
+    for (int32_t l = 0; l < L; ++l) off[l + 1] += off[l];
+    std::vector<int32_t> fill(off, off + L);
+    for (int64_t s = 0; s < S; ++s) ordered[fill[lev[s] - 1]++] = seq[s];
+    return L;
+}
+
 // The level schedule of a row sequence (one-based rows), made on the host from the pattern and
 // cached in the matrix; nullptr when it is switched off (FAIR_ART_LEVELS=0) or the sequence has
 // fewer than FAIR_ART_LEVEL_WIDTH (4) steps per level on average, where one wavefront walking the
@@ -595,29 +627,10 @@ const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
     A.IA.download(IA.data(), (size_t)A.m + 1);
     A.JA.download(JA.data(), (size_t)A.nnz);
     FAIR_HIP(hipStreamSynchronize(stream()));
-    std::vector<int32_t> last((size_t)A.n, 0), lev((size_t)S);
-    int32_t L = 0;
-    for (int64_t s = 0; s < S; ++s) {
-        const int32_t r = seq[s] - 1;
-        int32_t mx = 0;
-        for (int32_t k = IA[r]; k < IA[r + 1]; ++k) {
-            const int32_t j = JA[k];
-            if ((uint32_t)j < (uint32_t)A.n && last[j] > mx) mx = last[j];
-        }
-        ++mx;
-        for (int32_t k = IA[r]; k < IA[r + 1]; ++k) {
-            const int32_t j = JA[k];
-            if ((uint32_t)j < (uint32_t)A.n) last[j] = mx;
-        }
-        lev[s] = mx;
-        L = std::max(L, mx);
-    }
+    std::vector<int32_t> off((size_t)S + 1), ordered((size_t)S);
+    const int32_t L = levelize(A.n, IA.data(), JA.data(), 0, seq, S, ordered.data(), off.data());
+    off.resize((size_t)L + 1);
     c->nlev = L;
-    std::vector<int32_t> off((size_t)L + 1, 0), ordered((size_t)S);
-    for (int64_t s = 0; s < S; ++s) ++off[lev[s]];
-    for (int32_t l = 0; l < L; ++l) off[l + 1] += off[l];
-    std::vector<int32_t> fill(off.begin(), off.end() - 1);
-    for (int64_t s = 0; s < S; ++s) ordered[fill[lev[s] - 1]++] = seq[s];
     c->rows.alloc((size_t)S);
     c->off.alloc((size_t)L + 1);
     c->rows.upload(ordered.data(), (size_t)S);
@@ -1014,6 +1027,22 @@ extern "C" int fair_twin_art_uniform_bound(fair_csr *h, int32_t method, int32_t 
         ArtPlan P = plan_art(h, method, nk, Kin, order, nord, opts, d_nrA);
         *count = P.isrand ? ((int64_t)P.maxiter + 1) * (2 * (int64_t)P.nord + (P.rowprob ? 2 * (int64_t)P.nord : 0))
                           : 0;
+    });
+}
+
+// The level schedule of a row sequence over a host CSR pattern (one-based JA, as at the boundary);
+// host work only, no device call.
+extern "C" int fair_level_schedule(int32_t m, int32_t n, const int32_t *IA, const int32_t *JA, const int32_t *seq,
+                                   int64_t nsteps, int32_t *rows_out, int32_t *off_out, int32_t *nlev) {
+    return guard([&] {
+        FAIR_CHECK(IA && nlev && (nsteps == 0 || (seq && rows_out && off_out)) && m >= 0 && n >= 0 &&
+                       nsteps >= 0 && nsteps <= INT32_MAX, FAIR_ERR_ARG, "null argument or bad sizes");
+        FAIR_CHECK(IA[0] == 0 && (IA[m] == 0 || JA), FAIR_ERR_ARG, "IA(0) must be 0");
+        for (int32_t r = 0; r < m; ++r) FAIR_CHECK(IA[r + 1] >= IA[r], FAIR_ERR_ARG, "IA must be non-decreasing");
+        for (int64_t s = 0; s < nsteps; ++s)
+            FAIR_CHECK(seq[s] >= 1 && seq[s] <= m, FAIR_ERR_ARG, "row index out of range in the sequence");
+        *nlev = nsteps > 0 ? levelize(n, IA, JA, 1, seq, nsteps, rows_out, off_out) : 0;
+        if (nsteps == 0 && off_out) off_out[0] = 0;
     });
 }
 

@@ -213,6 +213,15 @@ int fair_art_steps(fair_csr *A, const int32_t *rows, int64_t nsteps, const doubl
 /* The same with device arrays (d_x in HBM), asynchronous; the rows must lie in 1..m. */
 int fair_art_steps_dev(fair_csr *A, const int32_t *d_rows, int64_t nsteps, const double *d_b,
                        const double *d_nrA, const fair_opts *opts, double *d_x);
+/* The level schedule the sweeps use, on host arrays (no device call).  No counterpart in the
+ * reference: it reorders the row steps of basic_sweep's loop (AIRMs.f90:813-858) without changing
+ * their result.  Step s gets level 1 + max(level of the last earlier step touching one of its
+ * columns), so the steps of one level touch disjoint columns and every column sees its steps in
+ * sequence order.  rows_out (nsteps): the sequence's rows sorted by (level, position);
+ * off_out (nsteps + 1 capacity): level offsets, *nlev + 1 of them.  IA zero-based offsets, JA
+ * one-based columns; columns outside 1..n are ignored. */
+int fair_level_schedule(int32_t m, int32_t n, const int32_t *IA, const int32_t *JA, const int32_t *seq,
+                        int64_t nsteps, int32_t *rows_out, int32_t *off_out, int32_t *nlev);
 
 /* ---- groups of independent SIRT problems (uneven batch) ------------------------------------
  * No counterpart in the reference (its to-do "blocking support", README.md:6).  Problems with
