@@ -132,7 +132,6 @@ struct LevelSched {
     // one kernel per level, captured once into a graph; its arguments come from params
     hipGraphExec_t graph = nullptr;
     DBuf<ArtParams> params;
-    ArtParams hparams{};
     LevelSched() = default;
     LevelSched(const LevelSched &) = delete;
     LevelSched &operator=(const LevelSched &) = delete;

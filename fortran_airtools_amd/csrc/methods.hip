@@ -574,6 +574,10 @@ __global__ void __launch_bounds__(LEVK_THREADS) k_art_level(const ArtParams *__r
 
 hipStream_t g_level_capture = nullptr;
 
+// Writes a replay's arguments into the schedule's parameter block, in stream order (launch
+// arguments are copied at launch, so no host buffer has to outlive the call).
+__global__ void k_set_art_params(ArtParams *P, ArtParams v) { *P = v; }
+
 int env_flag(const char *name, int dflt) {
     const char *v = getenv(name);
     return v && *v ? atoi(v) : dflt;
@@ -668,8 +672,8 @@ void graph_art_levels(const Csr &A, LevelSched *s, double *x, const double *b, c
         FAIR_HIP(ie);
         s->graph = ex;
     }
-    s->hparams = ArtParams{x, b, nrA, relax, box.lb, box.ub, box.lwrb, box.upb};
-    FAIR_HIP(hipMemcpyAsync(s->params.p, &s->hparams, sizeof(ArtParams), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_set_art_params, dim3(1), dim3(1), 0, st, s->params.p,
+                       ArtParams{x, b, nrA, relax, box.lb, box.ub, box.lwrb, box.upb});
     FAIR_HIP(hipGraphLaunch(s->graph, st));
     launch_check("art level graph");
 }
@@ -686,8 +690,9 @@ void launch_art_levels(const Csr &A, const LevelSched *s0, double *x0, const Lev
         if (env_flag("FAIR_DEBUG", 0))
             fprintf(stderr, "[fair] level sweep (graph): %zu steps in %d levels%s\n", s0->seq.size(), s0->nlev,
                     s1 ? " (two chains)" : "");
-        if (!s1) {
+        if (!s1 || s1 == s0) {   // one chain, or two chains of one sequence: one after the other
             graph_art_levels(A, const_cast<LevelSched *>(s0), x0, b, nrA, relax, box, stream());
+            if (s1) graph_art_levels(A, const_cast<LevelSched *>(s1), x1, b, nrA, relax, box, stream());
             return;
         }
         // two independent chains (twin_art): the second replays on a side stream alongside the

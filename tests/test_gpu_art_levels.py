@@ -123,6 +123,23 @@ def test_twin_art_levels_equal_sweep(monkeypatch, capfd, rule, method):
         np.testing.assert_array_equal(np.asarray(a), np.asarray(c))
 
 
+@pytest.mark.parametrize("order", [[90], [80, 81, 80], [70, 71, 72, 71, 70]], ids=["one_row", "aba", "abcba"])
+def test_twin_art_palindromic_order(monkeypatch, capfd, order):
+    """A palindromic order gives both chains the same sequence, hence one cached schedule."""
+    _, _, am = fair()
+    A = fan(80, 121, 72)
+    b = rhs(A, 11)
+    op = opts(am, relax=1.0, lb=0.0)
+    rows = np.asarray(order, np.int32)
+    rows = rows[A.row_norms(2)[rows - 1] > 0]
+    assert len(rows) == len(order)
+    lev, seq, used = both_paths(monkeypatch, capfd,
+                                lambda: am.twin_art("kaczmarz", "errorgauge", A, b, [3, 6], order=rows, options=op))
+    assert used
+    for a, c in zip(lev, seq):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(c))
+
+
 def sequences(m, rng):
     out = {f"random_{k}": rng.integers(1, m + 1, k) for k in (1, 2, 15, 16, 17, 33, 301, 2000)}
     out["repeat_one_row"] = np.full(37, m // 2)
