@@ -514,18 +514,18 @@ __global__ void __launch_bounds__(LEV_THREADS) k_art_levels(LevelChains ch, cons
 constexpr int LEVK_THREADS = WAVE;
 
 __global__ void __launch_bounds__(LEVK_THREADS) k_art_level(const ArtParams *__restrict__ P,
-                                                            const int32_t *__restrict__ rows, int32_t s0,
-                                                            int32_t s1, const int32_t *__restrict__ IA,
-                                                            const int32_t *__restrict__ JA,
+                                                            const int4 *__restrict__ desc, int32_t s0,
+                                                            int32_t s1, const int32_t *__restrict__ JA,
                                                             const double *__restrict__ vA, int32_t ncols) {
     const int lane = threadIdx.x;
     const int32_t s = s0 + (int32_t)blockIdx.x;
     if (s >= s1) return;
+    // the step's row and entry range in one load (the schedule holds them in level order)
+    const int4 d = desc[s];
+    const int32_t r = d.x, beg = d.y, end = d.z;
     double *x = P->x;
     const double relax = P->relax, lb = P->lb, ub = P->ub;
     const int lwrb = P->lwrb, upb = P->upb;
-    const int32_t r = rows[s] - 1;
-    const int32_t beg = IA[r], end = IA[r + 1];
     const double br = P->b[r], wr = P->nrA[r];
     int32_t jc[ART_R];
     double vc[ART_R], xv[ART_R];
@@ -637,8 +637,15 @@ const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
     c->nlev = L;
     c->rows.alloc((size_t)S);
     c->off.alloc((size_t)L + 1);
+    std::vector<int4> desc((size_t)S);
+    for (int64_t s = 0; s < S; ++s) {
+        const int32_t r = ordered[s] - 1;
+        desc[s] = make_int4(r, IA[r], IA[r + 1], 0);
+    }
+    c->desc.alloc(4 * (size_t)S);
     c->rows.upload(ordered.data(), (size_t)S);
     c->off.upload(off.data(), (size_t)L + 1);
+    FAIR_HIP(hipMemcpyAsync(c->desc.p, desc.data(), sizeof(int4) * S, hipMemcpyHostToDevice, stream()));
     FAIR_HIP(hipStreamSynchronize(stream()));
     c->hoff = std::move(off);
     if (A.lev.size() >= 6) A.lev.erase(A.lev.begin());
@@ -661,7 +668,7 @@ void graph_art_levels(const Csr &A, LevelSched *s, double *x, const double *b, c
         for (int32_t l = 0; l < s->nlev; ++l) {
             const int32_t w = s->hoff[l + 1] - s->hoff[l];
             hipLaunchKernelGGL(k_art_level, dim3((unsigned)w), dim3(LEVK_THREADS), 0, g_level_capture, s->params.p,
-                               s->rows.p, s->hoff[l], s->hoff[l + 1], A.IA.p, A.JA.p, A.vA.p, A.n);
+                               (const int4 *)s->desc.p, s->hoff[l], s->hoff[l + 1], A.JA.p, A.vA.p, A.n);
         }
         hipError_t le = hipGetLastError();
         FAIR_HIP(hipStreamEndCapture(g_level_capture, &g));
