@@ -154,6 +154,7 @@ struct Csr {
     int32_t max_row = 0;     // longest row
     Blocked blk;             // LDS-staged layout (built for ray-traced matrices)
     std::vector<std::unique_ptr<LevelSched>> lev;   // Kaczmarz level schedules, oldest first
+    std::vector<int32_t> hIA, hJA;                  // host copy of the pattern, made for level planning
 };
 
 }  // namespace fair

@@ -625,14 +625,26 @@ const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
     for (auto &c : A.lev)
         if ((int64_t)c->seq.size() == S && std::equal(c->seq.begin(), c->seq.end(), seq))
             return S >= width * c->nlev ? c.get() : nullptr;
+    if (A.hIA.empty()) {
+        A.hIA.resize((size_t)A.m + 1);
+        A.hJA.resize((size_t)std::max<int64_t>(A.nnz, 1));
+        A.IA.download(A.hIA.data(), (size_t)A.m + 1);
+        A.JA.download(A.hJA.data(), (size_t)A.nnz);
+        FAIR_HIP(hipStreamSynchronize(stream()));
+    }
+    const int32_t *IA = A.hIA.data(), *JA = A.hJA.data();
+    std::vector<int32_t> off((size_t)S + 1), ordered((size_t)S);
+    // a long sequence whose first eighth is too narrow is not planned in full (random orders,
+    // e.g. row-probability draws that change from call to call, stay about 2 steps per level;
+    // a cyclic order's prefix is already wider than its threshold)
+    const int64_t P = std::max<int64_t>(4096, S / 8);
+    if (P < S) {
+        const int32_t Lp = levelize(A.n, IA, JA, 0, seq, P, ordered.data(), off.data());
+        if (P < width * Lp) return nullptr;
+    }
     auto c = std::make_unique<LevelSched>();
     c->seq.assign(seq, seq + S);
-    std::vector<int32_t> IA((size_t)A.m + 1), JA((size_t)std::max<int64_t>(A.nnz, 1));
-    A.IA.download(IA.data(), (size_t)A.m + 1);
-    A.JA.download(JA.data(), (size_t)A.nnz);
-    FAIR_HIP(hipStreamSynchronize(stream()));
-    std::vector<int32_t> off((size_t)S + 1), ordered((size_t)S);
-    const int32_t L = levelize(A.n, IA.data(), JA.data(), 0, seq, S, ordered.data(), off.data());
+    const int32_t L = levelize(A.n, IA, JA, 0, seq, S, ordered.data(), off.data());
     off.resize((size_t)L + 1);
     c->nlev = L;
     c->rows.alloc((size_t)S);
