@@ -484,9 +484,10 @@ def bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
 
 def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     """Kaczmarz at N=256 (fan beam 360 x 363 rays, three-phase phantom, 1 % noise, seed 5), box
-    [0,1], 20 sweeps: randomized Kaczmarz with row-norm sampling (the reference's random_number
-    draws supplied from a seeded generator), symmetric Kaczmarz (K=20: 10 forward + reverse
-    pairs) and cyclic Kaczmarz (K=20), and twin_art with symmetric Kaczmarz and the errorgauge rule.
+    [0,1], 20 sweeps: randomized Kaczmarz with row-norm sampling and with uniform (scrambled)
+    order (the reference's random_number draws supplied from a seeded generator), symmetric
+    Kaczmarz (K=20: 10 forward + reverse pairs) and cyclic Kaczmarz (K=20), and twin_art with
+    symmetric Kaczmarz and the errorgauge rule.
     Each GPU result is checked against the oracle run on the same draws; the reference's own art
     and twin_art (one host thread, as they are serial) are timed on the same problem."""
     g = cf.C3
@@ -501,7 +502,8 @@ def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     build_oracle()
     orc = Oracle()
     rows = []
-    for method, rowprob in (("randkaczmarz", True), ("symkaczmarz", False), ("kaczmarz", False)):
+    for method, rowprob in (("randkaczmarz", True), ("randkaczmarz", False), ("symkaczmarz", False),
+                            ("kaczmarz", False)):
         o.rowprobability = rowprob
         u = None
         if method == "randkaczmarz":
@@ -528,7 +530,8 @@ def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
             t0 = time.perf_counter()
             ref.art(0, method, b, [20], lb=0.0, ub=1.0, rowprob=rowprob)
             t_ref = time.perf_counter() - t0
-        log(f"{method}: GPU {t_gpu:.3f} s ({steps / t_gpu:.3g} rows/s), rel. error to oracle {err:.2e}"
+        log(f"{method}{' (row norms)' if rowprob else ''}: GPU {t_gpu:.3f} s ({steps / t_gpu:.3g} rows/s), "
+            f"rel. error to oracle {err:.2e}"
             + (f"; reference on one host thread {t_ref:.3f} s" if t_ref else ""))
         rows.append({"method": method, "rowprobability": rowprob, "sweeps": 20, "row_steps": steps,
                      "gpu_s": round(t_gpu, 4), "rows_per_s": round(steps / t_gpu, 1),
