@@ -373,9 +373,12 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, con
 
 // ---- level-parallel sweep ----------------------------------------------------------------
 // The same row steps as k_art_sweep, in an order that keeps every pixel's update sequence: the
-// steps of one level (LevelSched) touch disjoint pixels, so the LEV_WAVES wavefronts of one
-// workgroup take them side by side (wavefront w: steps off[l] + w, + LEV_WAVES, ...) and a
-// workgroup barrier separates the levels.  One workgroup per chain keeps each chain's stores and
+// steps of one level (LevelSched) touch disjoint pixels.  Two executors: one launch per level
+// (k_art_level below, replayed from a captured graph; the default), and this one, in which the
+// LEV_WAVES wavefronts of one workgroup take a level's steps side by side (wavefront w: steps
+// off[l] + w, + LEV_WAVES, ...) and a workgroup barrier separates the levels
+// (FAIR_ART_LEVEL_MODE=workgroup; 13.3 us per level against 3.7 us at N = 256, as one CU issues
+// every gather and store of a level).  One workgroup per chain keeps each chain's stores and
 // loads on one CU, where the barrier orders them.  Each row is computed exactly as k_art_sweep
 // computes it (entries of lane q: beg + q + 64 t in increasing t, the same DPP reduction and
 // update), so the iterate equals k_art_sweep's bit for bit.  A wavefront loads the entries of its
@@ -618,7 +621,8 @@ int32_t levelize(int32_t n, const int32_t *IA, const int32_t *JA, int32_t base, 
 // The level schedule of a row sequence (one-based rows), made on the host from the pattern and
 // cached in the matrix; nullptr when it is switched off (FAIR_ART_LEVELS=0) or the sequence has
 // fewer than FAIR_ART_LEVEL_WIDTH (4) steps per level on average, where one wavefront walking the
-// sequence is as fast.
+// sequence is as fast.  A long sequence whose first eighth is already that narrow is rejected
+// before it is planned in full.
 const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
     if (S <= 0 || S > INT32_MAX || A.n <= 0 || !env_flag("FAIR_ART_LEVELS", 1)) return nullptr;
     const int64_t width = std::max(1, env_flag("FAIR_ART_LEVEL_WIDTH", 4));
