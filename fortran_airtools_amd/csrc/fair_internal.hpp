@@ -127,6 +127,7 @@ struct ArtParams {
 struct LevelSched {
     std::vector<int32_t> seq;   // the one-based row sequence it was made for
     std::vector<int32_t> hoff;  // host copy of off
+    std::vector<int32_t> hmaxlen;  // longest row of each level
     DBuf<int32_t> rows, off;
     DBuf<int32_t> desc;         // int4 per step in level order: {row (zero-based), IA(row), IA(row+1), 0}
     int32_t nlev = 0;

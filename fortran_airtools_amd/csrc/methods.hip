@@ -515,7 +515,13 @@ __global__ void __launch_bounds__(LEV_THREADS) k_art_levels(LevelChains ch, cons
 // are ordered by the stream (a captured graph), which also makes each level's stores visible to
 // the next.  Each row is computed as k_art_sweep computes it.
 constexpr int LEVK_THREADS = WAVE;
+// R entries per lane are held in registers (64 R per row; longer rows finish in a loop).  Each
+// level is instantiated for its longest row: R = 8 up to 512 entries, R = 16 (a wavefront alone
+// in its workgroup has the registers) above, so the rows of an N = 512 image (up to about 1000
+// entries) need no loop while short rows do not pay for empty slots.  A lane adds its entries in
+// increasing order either way, so the sums do not depend on R.
 
+template <int LEVK_R>
 __global__ void __launch_bounds__(LEVK_THREADS) k_art_level(const ArtParams *__restrict__ P,
                                                             const int4 *__restrict__ desc, int32_t s0,
                                                             int32_t s1, const int32_t *__restrict__ JA,
@@ -530,10 +536,10 @@ __global__ void __launch_bounds__(LEVK_THREADS) k_art_level(const ArtParams *__r
     const double relax = P->relax, lb = P->lb, ub = P->ub;
     const int lwrb = P->lwrb, upb = P->upb;
     const double br = P->b[r], wr = P->nrA[r];
-    int32_t jc[ART_R];
-    double vc[ART_R], xv[ART_R];
+    int32_t jc[LEVK_R];
+    double vc[LEVK_R], xv[LEVK_R];
 #pragma unroll
-    for (int u = 0; u < ART_R; ++u) {
+    for (int u = 0; u < LEVK_R; ++u) {
         const int32_t k = beg + lane + WAVE * u;
         const int32_t kk = k < end ? k : 0;
         const int32_t jv = JA[kk];
@@ -542,22 +548,22 @@ __global__ void __launch_bounds__(LEVK_THREADS) k_art_level(const ArtParams *__r
         vc[u] = k < end ? vv : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < ART_R; ++u) {
+    for (int u = 0; u < LEVK_R; ++u) {
         const bool ok = (uint32_t)jc[u] < (uint32_t)ncols;
         const double xx = x[ok ? jc[u] : 0];
         xv[u] = ok ? xx : 0.0;
     }
     double dot = 0.0;
 #pragma unroll
-    for (int u = 0; u < ART_R; ++u) dot += xv[u] * vc[u];
-    for (int32_t k = beg + lane + WAVE * ART_R; k < end; k += WAVE) {
+    for (int u = 0; u < LEVK_R; ++u) dot += xv[u] * vc[u];
+    for (int32_t k = beg + lane + WAVE * LEVK_R; k < end; k += WAVE) {
         const int j = JA[k];
         if ((uint32_t)j < (uint32_t)ncols) dot += x[j] * vA[k];
     }
     dot = wave_sum_dpp(dot);
     const double coeff = (relax * (br - dot)) / wr;
 #pragma unroll
-    for (int u = 0; u < ART_R; ++u) {
+    for (int u = 0; u < LEVK_R; ++u) {
         if ((uint32_t)jc[u] < (uint32_t)ncols) {
             double v = xv[u] + coeff * vc[u];
             if (lwrb) v = fmax_ref(v, lb);
@@ -565,7 +571,7 @@ __global__ void __launch_bounds__(LEVK_THREADS) k_art_level(const ArtParams *__r
             x[jc[u]] = v;
         }
     }
-    for (int32_t k = beg + lane + WAVE * ART_R; k < end; k += WAVE) {
+    for (int32_t k = beg + lane + WAVE * LEVK_R; k < end; k += WAVE) {
         const int j = JA[k];
         if ((uint32_t)j >= (uint32_t)ncols) continue;
         double v = x[j] + coeff * vA[k];
@@ -658,6 +664,10 @@ const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
         const int32_t r = ordered[s] - 1;
         desc[s] = make_int4(r, IA[r], IA[r + 1], 0);
     }
+    c->hmaxlen.assign((size_t)L, 0);
+    for (int32_t l = 0; l < L; ++l)
+        for (int32_t s = off[l]; s < off[l + 1]; ++s)
+            c->hmaxlen[l] = std::max(c->hmaxlen[l], desc[s].z - desc[s].y);
     c->desc.alloc(4 * (size_t)S);
     c->rows.upload(ordered.data(), (size_t)S);
     c->off.upload(off.data(), (size_t)L + 1);
@@ -683,8 +693,14 @@ void graph_art_levels(const Csr &A, LevelSched *s, double *x, const double *b, c
         FAIR_HIP(hipStreamBeginCapture(g_level_capture, hipStreamCaptureModeThreadLocal));
         for (int32_t l = 0; l < s->nlev; ++l) {
             const int32_t w = s->hoff[l + 1] - s->hoff[l];
-            hipLaunchKernelGGL(k_art_level, dim3((unsigned)w), dim3(LEVK_THREADS), 0, g_level_capture, s->params.p,
-                               (const int4 *)s->desc.p, s->hoff[l], s->hoff[l + 1], A.JA.p, A.vA.p, A.n);
+            if (s->hmaxlen[l] <= 8 * WAVE)
+                hipLaunchKernelGGL(k_art_level<8>, dim3((unsigned)w), dim3(LEVK_THREADS), 0, g_level_capture,
+                                   s->params.p, (const int4 *)s->desc.p, s->hoff[l], s->hoff[l + 1], A.JA.p, A.vA.p,
+                                   A.n);
+            else
+                hipLaunchKernelGGL(k_art_level<16>, dim3((unsigned)w), dim3(LEVK_THREADS), 0, g_level_capture,
+                                   s->params.p, (const int4 *)s->desc.p, s->hoff[l], s->hoff[l + 1], A.JA.p, A.vA.p,
+                                   A.n);
         }
         hipError_t le = hipGetLastError();
         FAIR_HIP(hipStreamEndCapture(g_level_capture, &g));
