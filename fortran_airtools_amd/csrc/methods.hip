@@ -254,10 +254,10 @@ __global__ void __launch_bounds__(512) k_gd_decide(const double *part, GdState *
 // every lane; coeff = relax*(b_i - dot)/nrA_i (:842); then x(JA) += coeff*vA with max(lb) /
 // min(ub) on the touched entries (sparse_matrices.f90:808-820).  Column indices are unique
 // within a row, so each entry has one owner lane.
-// Latency hiding: the entries of row s+1 (up to 64*ART_R of them) are loaded into registers while
-// row s is processed -- issued after row s's x gathers, so the in-order load counter lets those
-// gathers complete first -- and row descriptors run two steps ahead.  Longer rows load the
-// remainder inline.
+// Latency hiding: the entries of row s+1 (up to 64*R of them, R the template argument) are loaded
+// into registers while row s is processed -- issued after row s's x gathers, so the in-order load
+// counter lets those gathers complete first -- and row descriptors run two steps ahead.  Longer
+// rows load the remainder inline.
 constexpr int SWEEP_THREADS = WAVE;
 constexpr int ART_R = 8;
 
@@ -287,6 +287,7 @@ struct SweepChains {
     double *x[2];
 };
 
+template <int R>
 __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, const int32_t *__restrict__ IA,
                                                              const int32_t *__restrict__ JA,
                                                              const double *__restrict__ vA,
@@ -301,11 +302,11 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, con
     if (nsteps <= 0) return;
     ArtRow cur = art_row(rows, 0, nsteps, IA, b, nrA);
     ArtRow nxt = art_row(rows, 1, nsteps, IA, b, nrA);
-    int32_t jc[ART_R];
-    double vc[ART_R];
+    int32_t jc[R];
+    double vc[R];
     // JA / vA always have CSR_PAD spare entries, so index 0 is a valid address even for nnz = 0
 #pragma unroll
-    for (int u = 0; u < ART_R; ++u) {
+    for (int u = 0; u < R; ++u) {
         const int32_t k = cur.beg + lane + WAVE * u;
         const int32_t kk = k < cur.end ? k : 0;
         const int32_t jv = JA[kk];
@@ -315,18 +316,18 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, con
     }
     for (int64_t s = 0; s < nsteps; ++s) {
         // x gathers of this row (after the previous row's stores)
-        double xv[ART_R];
+        double xv[R];
 #pragma unroll
-        for (int u = 0; u < ART_R; ++u) {
+        for (int u = 0; u < R; ++u) {
             const bool ok = (uint32_t)jc[u] < (uint32_t)ncols;
             const double xx = x[ok ? jc[u] : 0];
             xv[u] = ok ? xx : 0.0;
         }
         // prefetch: entries of the next row, descriptor of the row after
-        int32_t jn[ART_R];
-        double vn[ART_R];
+        int32_t jn[R];
+        double vn[R];
 #pragma unroll
-        for (int u = 0; u < ART_R; ++u) {
+        for (int u = 0; u < R; ++u) {
             const int32_t k = nxt.beg + lane + WAVE * u;
             const int32_t kk = k < nxt.end ? k : 0;
             const int32_t jv = JA[kk];
@@ -337,15 +338,15 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, con
         const ArtRow nn = art_row(rows, s + 2, nsteps, IA, b, nrA);
         double dot = 0.0;
 #pragma unroll
-        for (int u = 0; u < ART_R; ++u) dot += xv[u] * vc[u];
-        for (int32_t k = cur.beg + lane + WAVE * ART_R; k < cur.end; k += WAVE) {
+        for (int u = 0; u < R; ++u) dot += xv[u] * vc[u];
+        for (int32_t k = cur.beg + lane + WAVE * R; k < cur.end; k += WAVE) {
             const int j = JA[k];
             if ((uint32_t)j < (uint32_t)ncols) dot += x[j] * vA[k];
         }
         dot = wave_sum_dpp(dot);
         const double coeff = (relax * (cur.b - dot)) / cur.w;
 #pragma unroll
-        for (int u = 0; u < ART_R; ++u) {
+        for (int u = 0; u < R; ++u) {
             if ((uint32_t)jc[u] < (uint32_t)ncols) {
                 double v = xv[u] + coeff * vc[u];
                 if (lwrb) v = fmax_ref(v, lb);
@@ -353,7 +354,7 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, con
                 x[jc[u]] = v;
             }
         }
-        for (int32_t k = cur.beg + lane + WAVE * ART_R; k < cur.end; k += WAVE) {
+        for (int32_t k = cur.beg + lane + WAVE * R; k < cur.end; k += WAVE) {
             const int j = JA[k];
             if ((uint32_t)j >= (uint32_t)ncols) continue;
             double v = x[j] + coeff * vA[k];
@@ -364,11 +365,23 @@ __global__ void __launch_bounds__(SWEEP_THREADS) k_art_sweep(SweepChains ch, con
         cur = nxt;
         nxt = nn;
 #pragma unroll
-        for (int u = 0; u < ART_R; ++u) {
+        for (int u = 0; u < R; ++u) {
             jc[u] = jn[u];
             vc[u] = vn[u];
         }
     }
+}
+
+// k_art_sweep with R entries per lane in registers: 8 (512 per row), or 16 for a matrix with
+// longer rows (an N = 512 image has rows of about 1000 entries), which then need no loop.  A lane
+// adds its entries in increasing order either way, so the iterate does not depend on R.
+template <class... Args>
+void launch_art_sweep(Csr &A, unsigned chains, Args... args) {
+    if (A.n_rows_nz < 0) build_row_list(A);
+    if (A.max_row > 8 * WAVE)
+        hipLaunchKernelGGL(k_art_sweep<16>, dim3(chains), dim3(SWEEP_THREADS), 0, stream(), args...);
+    else
+        hipLaunchKernelGGL(k_art_sweep<8>, dim3(chains), dim3(SWEEP_THREADS), 0, stream(), args...);
 }
 
 // ---- level-parallel sweep ----------------------------------------------------------------
@@ -1113,7 +1126,7 @@ extern "C" int fair_art_steps(fair_csr *h, const int32_t *rows, int64_t nsteps, 
         if (const LevelSched *lev = art_levels(h->A, rows, nsteps)) {
             launch_art_levels(h->A, lev, dx.p, nullptr, nullptr, db.p, dw.p, relax, box);
         } else {
-            hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
+            launch_art_sweep(h->A, 1,
                                SweepChains{{dr.p, nullptr}, {nsteps, 0}, {dx.p, nullptr}}, h->A.IA.p, h->A.JA.p,
                                h->A.vA.p, db.p, dw.p, n, relax, box.lwrb, box.lb, box.upb, box.ub);
             launch_check("art sweep");
@@ -1132,7 +1145,7 @@ extern "C" int fair_art_steps_dev(fair_csr *h, const int32_t *d_rows, int64_t ns
         const Box box = make_box(o);
         const double relax = o ? o->relaxpar : 1.0;
         if (nsteps > 0 && h->A.n > 0)
-            hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
+            launch_art_sweep(h->A, 1,
                                SweepChains{{d_rows, nullptr}, {nsteps, 0}, {d_x, nullptr}}, h->A.IA.p, h->A.JA.p,
                                h->A.vA.p, d_b, d_nrA, h->A.n, relax, box.lwrb, box.lb, box.upb, box.ub);
         launch_check("art sweep");
@@ -1222,7 +1235,7 @@ extern "C" int fair_twin_art(fair_csr *h, int32_t method, int32_t stoprule, cons
             }
             dsx.upload(sx.data(), per_chain);
             dsy.upload(sy.data(), per_chain);
-            hipLaunchKernelGGL(k_art_sweep, dim3(2), dim3(SWEEP_THREADS), 0, stream(),
+            launch_art_sweep(h->A, 2,
                                SweepChains{{dsx.p, dsy.p}, {(int64_t)per_chain, (int64_t)per_chain}, {x.p, y.p}},
                                h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p, n, relax, box.lwrb, box.lb, box.upb,
                                box.ub);
@@ -1388,7 +1401,7 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
             launch_art_levels(h->A, lev, dx.p, nullptr, nullptr, db.p, d_nrA.p, relax, box);
         } else if (steps_per_iter > 0) {
             dseq.upload(seq.data(), steps_per_iter);
-            hipLaunchKernelGGL(k_art_sweep, dim3(1), dim3(SWEEP_THREADS), 0, stream(),
+            launch_art_sweep(h->A, 1,
                                SweepChains{{dseq.p, nullptr}, {(int64_t)steps_per_iter, 0}, {dx.p, nullptr}},
                                h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p, h->A.n, relax, box.lwrb, box.lb,
                                box.upb, box.ub);
