@@ -79,6 +79,8 @@ def cpu_baseline(iters: int, method: str = "sart", geom=None) -> dict | None:
         log(f"cpu baseline failed (status {r.returncode}):", r.stderr[-2000:])
         return None
     d = json.loads(r.stdout.strip().splitlines()[-1])
+    if d is None:                       # no compiled reference for this method
+        return None
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -486,8 +488,9 @@ def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     """Kaczmarz at N=256 (fan beam 360 x 363 rays, three-phase phantom, 1 % noise, seed 5), box
     [0,1], 20 sweeps: randomized Kaczmarz with row-norm sampling and with uniform (scrambled)
     order (the reference's random_number draws supplied from a seeded generator), symmetric
-    Kaczmarz (K=20: 10 forward + reverse pairs) and cyclic Kaczmarz (K=20), and twin_art with
-    symmetric Kaczmarz and the errorgauge rule.
+    Kaczmarz (K=20: 10 forward + reverse pairs) and cyclic Kaczmarz (K=20), twin_art with
+    symmetric Kaczmarz and the errorgauge rule, and cyclic Kaczmarz (4 sweeps) on the N=512 SART
+    problem.
     Each GPU result is checked against the oracle run on the same draws; the reference's own art
     and twin_art (one host thread, as they are serial) are timed on the same problem."""
     g = cf.C3
@@ -561,6 +564,30 @@ def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     rows.append({"method": "twin_art symkaczmarz errorgauge", "kin": 20, "final_it": fit, "final_it_oracle": fio,
                  "gpu_s": round(t_gpu, 4), "relerr_vs_oracle": err, "within_1e-12": err <= 1e-12,
                  "reference_host_s": None if t_ref is None else round(t_ref, 4)})
+    # cyclic Kaczmarz on the N=512 SART problem (720 x 725 rays, rows up to about 1000 entries):
+    # 4 sweeps against the oracle, the reference's sweep time from its own art on one host thread
+    g2 = cf.C2
+    A2 = sg.fanlineartomo(sg.Data(n=g2.n, p=g2.p, nth=g2.nth, r=g2.r, dw=g2.dw, sd=g2.sd, theta=g2.theta()))
+    b2 = A2.matvecmul(cf.grains(g2.n, 200, 2))
+    o.rowprobability = False
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    airms.art("kaczmarz", A2, b2, [1], options=o)                              # schedule + graph
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    X2 = airms.art("kaczmarz", A2, b2, [4], options=o)
+    t_gpu = time.perf_counter() - t0
+    I2, J2, v2 = A2.components()
+    Xo2, _ = orc.art("kaczmarz", I2, J2, v2, A2.n, b2, [4], lb=0.0, ub=1.0)
+    err = float(np.linalg.norm(X2[0] - Xo2[0]) / np.linalg.norm(Xo2[0]))
+    ref2 = None if a.no_cpu else cpu_baseline(1, "kaczmarz", geom=g2)
+    t_ref = None if ref2 is None else 4.0 / ref2["value"]
+    log(f"kaczmarz N=512: GPU {t_gpu:.3f} s for 4 sweeps (first call, 1 sweep: {t_first:.3f} s), rel. error to "
+        f"oracle {err:.2e}" + (f"; reference on one host thread {t_ref:.3f} s" if t_ref else ""))
+    rows.append({"method": "kaczmarz", "n": g2.n, "sweeps": 4, "gpu_s": round(t_gpu, 4),
+                 "first_call_s": round(t_first, 4), "first_call_sweeps": 1, "relerr_vs_oracle": err,
+                 "within_1e-12": err <= 1e-12, "reference_host_s": None if t_ref is None else round(t_ref, 4),
+                 "reference_sample": None if ref2 is None else ref2["sample"]})
     out = {"metric": METRIC, "mode": "kaczmarz", "unit": "rows/s", "dtype": "f64",
            "config": {"workload": "N=256 fan beam 360 x 363 rays, Kaczmarz 20 sweeps, box [0,1]",
                       "nnz": A.nnz, "nonempty_rows": nonempty},

@@ -1,6 +1,8 @@
 """CPU baseline timing of the reference path (test infrastructure; bench.py's cpu_baseline leg).
 
-Times the reference's own `sirt` or `gradient_descent` (--method gd) (oracle/_ref, the reference's Fortran compiled with flang
+Times the reference's own `sirt`, `gradient_descent` (--method gd) or cyclic `art` (--method
+kaczmarz, one host thread as the method is serial; its automatic arrays need a stack above 8 MB at
+N=512, which the caller raises) (oracle/_ref, the reference's Fortran compiled with flang
 -O3 -fopenmp; kind "reference") on the host cores, or the C restatement (kind "port",
 single-threaded) when the compiled reference is absent.  The matrix comes from the C oracle's
 ray tracer (bit-identical to the reference's fanlineartomo) so that the bounded sample does not
@@ -51,14 +53,20 @@ def main() -> None:
     rng = np.random.default_rng(2)
     b = o.matvec(IA, JA, vA, rng.random(n))
     gd = a.method == "gd"
+    kz = a.method == "kaczmarz"
+    if kz and not Reference.available():
+        print(json.dumps(None))
+        return
     if Reference.available():
         ref = Reference()
-        kind, cores = "reference", a.threads
+        kind, cores = "reference", 1 if kz else a.threads
 
         def run(k):
             ref.set(0, IA, JA, vA, n)
             t = time.perf_counter()
-            if gd:
+            if kz:                       # k + 1 sweeps: the first carries the set-up
+                ref.art(0, "kaczmarz", b, [k + 1], lb=0.0, ub=1.0)
+            elif gd:
                 ref.rng_seed(1)          # the same normest start (and iteration count) every run
                 ref.gd(0, b, k, tol=0.0, lb=0.0, ub=1.0)
             else:
@@ -90,7 +98,8 @@ def main() -> None:
                       "nnz": int(len(JA)), "oracle_build_s": t_build,
                       "setup_s": t_setup,
                       "sample": f"{a.method} on N={a.n} fan {a.nth}x{a.p}: "
-                                f"(t({k} it) - t(0 it)) / {k}, {kind} on {cores} thread(s)"}))
+                                f"(t({k} it) - t(0 it)) / {k}, {kind} on {cores} thread(s)"
+                                + (" (an iteration is one sweep; t(k) runs k + 1 sweeps)" if kz else "")}))
 
 
 if __name__ == "__main__":
