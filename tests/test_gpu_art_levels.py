@@ -1,5 +1,6 @@
-"""The level-parallel Kaczmarz sweep (k_art_levels) against the one-wavefront sweep (k_art_sweep)
-bit for bit, and against the oracle to the 1e-12 bar.
+"""The level-parallel Kaczmarz sweeps (k_art_level, one launch per level; k_art_levels, one
+workgroup) against the one-wavefront sweep (k_art_sweep) bit for bit, and against the oracle to
+the 1e-12 bar.
 
 A level schedule sorts a row sequence so that the steps of one level touch disjoint pixels; every
 pixel then sees its updates in sequence order and each row is computed as the one-wavefront
@@ -8,7 +9,8 @@ symkaczmarz, a caller's order) and twin_art (both stopping rules) on fan and par
 with box, relaxation and damping; basic_sweep on explicit sequences (short, repeated rows, pairs,
 back-and-forth, random) with the width threshold lowered to 1 so that narrow schedules run on
 the level path too; rows above the 512 entries a wavefront keeps in registers; the N = 256 fan
-of the Kaczmarz benchmark (about 44 steps per level).  Every test runs with both executors of a
+of the Kaczmarz benchmark (about 44 steps per level) and the N = 512 SART problem (85 steps per
+level, rows up to about 1000 entries).  Every test runs with both executors of a
 schedule: one workgroup of 16 wavefronts with a barrier between levels, and one launch per level
 captured into a graph (FAIR_ART_LEVEL_MODE=graph).  FAIR_DEBUG=1 makes the library print one line
 per level sweep, which shows that the level path ran."""
@@ -208,3 +210,22 @@ def test_benchmark_fan_default_width(monkeypatch, capfd):
     lev, seq, used = both_paths(monkeypatch, capfd, lambda: am.art("symkaczmarz", A, b, [2], options=op), width="4")
     assert used
     np.testing.assert_array_equal(lev, seq)
+
+
+def test_sart_problem_n512(monkeypatch, capfd, oracle):
+    """N = 512, 720 x 725 rays (rows up to about 1000 entries, levels built with 16 entries per
+    lane): two cyclic sweeps, bit for bit against the one-wavefront sweep and to 1e-12 against the
+    oracle."""
+    _, _, am = fair()
+    from fortran_airtools_amd import configs as cf
+    sg, _, _ = fair()
+    g = cf.C2
+    A = sg.fanlineartomo(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta()))
+    b = rhs(A, 12)
+    op = opts(am, relax=1.0, lb=0.0, ub=1.0)
+    lev, seq, used = both_paths(monkeypatch, capfd, lambda: am.art("kaczmarz", A, b, [2], options=op), width="4")
+    assert used
+    np.testing.assert_array_equal(lev, seq)
+    IA, JA, vA = A.components()
+    want, _ = oracle.art("kaczmarz", IA, JA, vA, A.n, b, [2], lb=0.0, ub=1.0)
+    assert_close_1e12(lev, want)
