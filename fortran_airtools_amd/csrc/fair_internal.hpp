@@ -155,7 +155,8 @@ struct Csr {
     int32_t max_row = 0;     // longest row
     Blocked blk;             // LDS-staged layout (built for ray-traced matrices)
     std::vector<std::unique_ptr<LevelSched>> lev;   // Kaczmarz level schedules, oldest first
-    std::vector<int32_t> hIA, hJA;                  // host copy of the pattern, made for level planning
+    std::vector<int32_t> hIA, hJA;                  // host copy of the pattern for level planning
+                                                    // (kept up to 64 M entries)
 };
 
 }  // namespace fair

@@ -655,6 +655,17 @@ const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
         A.JA.download(A.hJA.data(), (size_t)A.nnz);
         FAIR_HIP(hipStreamSynchronize(stream()));
     }
+    // the host copy is kept for later plans up to 64 M entries (256 MB) and released after this
+    // plan above that
+    struct Release {
+        Csr &A;
+        ~Release() {
+            if (A.nnz > (int64_t(1) << 26)) {
+                std::vector<int32_t>().swap(A.hIA);
+                std::vector<int32_t>().swap(A.hJA);
+            }
+        }
+    } release{A};
     const int32_t *IA = A.hIA.data(), *JA = A.hJA.data();
     std::vector<int32_t> off((size_t)S + 1), ordered((size_t)S);
     // a long sequence whose first eighth is too narrow is not planned in full (random orders,
