@@ -92,7 +92,7 @@ def cpu_baseline(iters: int, method: str = "sart", geom=None) -> dict | None:
         pass
     return {"value": round(d["iter_per_s"], 4), "unit": "iter/s", "cores": d["cores"],
             "kind": d["kind"], "sample": d["sample"] + f"; {cpu_model}",
-            "gbs": round(d["gbs"], 2)}
+            "gbs": None if d["gbs"] is None else round(d["gbs"], 2)}
 
 
 def main() -> int:

@@ -94,7 +94,7 @@ def main() -> None:
     per_iter = max(t_k - t_setup, 1e-9) / k
     bytes_iter = 24 * len(JA) + 4 * (len(IA) - 1 + n + 2) + 32 * (len(IA) - 1) + 32 * n
     print(json.dumps({"kind": kind, "cores": cores, "iter_per_s": 1.0 / per_iter,
-                      "sec_per_iter": per_iter, "gbs": bytes_iter / per_iter / 1e9,
+                      "sec_per_iter": per_iter, "gbs": None if kz else bytes_iter / per_iter / 1e9,
                       "nnz": int(len(JA)), "oracle_build_s": t_build,
                       "setup_s": t_setup,
                       "sample": f"{a.method} on N={a.n} fan {a.nth}x{a.p}: "
