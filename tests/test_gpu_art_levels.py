@@ -229,3 +229,25 @@ def test_sart_problem_n512(monkeypatch, capfd, oracle):
     IA, JA, vA = A.components()
     want, _ = oracle.art("kaczmarz", IA, JA, vA, A.n, b, [2], lb=0.0, ub=1.0)
     assert_close_1e12(lev, want)
+
+
+def test_schedule_cache_eviction(monkeypatch, capfd):
+    """More sequences than the matrix caches (six): the oldest schedules and their graphs are
+    dropped and planned again when reused; every result equals the one-wavefront sweep's."""
+    _, _, am = fair()
+    A = parallel(96, 137, 60)
+    b = rhs(A, 13)
+    nrA = A.row_norms(2) ** 2
+    rows = np.nonzero(nrA > 0)[0].astype(np.int32) + 1
+    w = np.where(nrA > 0, nrA, 1.0)
+    x0 = np.random.default_rng(14).random(A.n)
+    seqs = [np.roll(rows, 37 * k) for k in range(8)] + [rows]
+    for k, seq in enumerate(seqs):
+        print("sequence", k, flush=True)
+
+        def run():
+            return am.basic_sweep(x0, A, b, seq, len(seq), False, False, 1.0, w, None, True, True, 1.0, 0.0)
+
+        lev, ref, used = both_paths(monkeypatch, capfd, run)
+        assert used
+        np.testing.assert_array_equal(lev, ref)
