@@ -204,13 +204,15 @@ void spmm_launch(const Csr &A, const float *X, int32_t k, int kc, Epi epi, hipSt
 }
 
 // Cached HIP graphs of whole batched runs, keyed by the run's arguments.
+// A captured run also bakes in handle-owned pointers (weights, FP32 values, layouts, workspaces):
+// whatever replaces them drops the handle's graphs.  The tuned widths are part of the key.
 struct GraphKey {
     const void *h, *B, *X;
-    int32_t method, k, maxits, upb, lwrb;
+    int32_t method, k, maxits, upb, lwrb, kc0, kc1;
     double relax, ub, lb;
     bool operator<(const GraphKey &o) const {
-        return std::tie(h, B, X, method, k, maxits, upb, lwrb, relax, ub, lb) <
-               std::tie(o.h, o.B, o.X, o.method, o.k, o.maxits, o.upb, o.lwrb, o.relax, o.ub, o.lb);
+        return std::tie(h, B, X, method, k, maxits, upb, lwrb, kc0, kc1, relax, ub, lb) <
+               std::tie(o.h, o.B, o.X, o.method, o.k, o.maxits, o.upb, o.lwrb, o.kc0, o.kc1, o.relax, o.ub, o.lb);
     }
 };
 std::map<GraphKey, hipGraphExec_t> g_graphs;
@@ -315,8 +317,13 @@ extern "C" int fair_sirt_batch_dev(fair_csr *h, int32_t method, const float *d_B
         make_f32(h->T);
         const int32_t m = h->A.m, n = h->A.n;
         const size_t need_r = (size_t)m * k, need_x = (size_t)n * k;
-        if (h->ws_rb.n < need_r) h->ws_rb.alloc(need_r);
-        if (h->ws_xb.n < need_x) h->ws_xb.alloc(need_x);
+        if (h->ws_rb.n < need_r || h->ws_xb.n < need_x) {
+            // captured graphs refer to the old workspaces
+            FAIR_HIP(hipStreamSynchronize(stream()));
+            if (h->graphs_used) drop_graphs(h);
+            if (h->ws_rb.n < need_r) h->ws_rb.alloc(need_r);
+            if (h->ws_xb.n < need_x) h->ws_xb.alloc(need_x);
+        }
         if (!use_graph) {
             batch_iterations(h, method, d_B, k, maxits, d_X, h->ws_rb.p, h->ws_xb.p, o, stream());
             launch_check("sirt batch");
@@ -324,7 +331,8 @@ extern "C" int fair_sirt_batch_dev(fair_csr *h, int32_t method, const float *d_B
         }
         // graph path: capture the whole run once on a private stream, replay afterwards
         if (!g_capture) FAIR_HIP(hipStreamCreateWithFlags(&g_capture, hipStreamNonBlocking));
-        GraphKey key{h, d_B, d_X, method, k, maxits, o ? o->upb : 0, o ? o->lwrb : 0,
+        GraphKey key{h,        d_B,          d_X,          method,       k,
+                     maxits,   o ? o->upb : 0, o ? o->lwrb : 0, h->spmm_kc[0], h->spmm_kc[1],
                      o ? o->relaxpar : 1.0, o ? o->ubound : 0.0, o ? o->lbound : 1.0};
         auto it = g_graphs.find(key);
         if (it == g_graphs.end()) {

@@ -154,7 +154,9 @@ struct Csr {
     int32_t n_rows_empty = 0;
     int32_t max_row = 0;     // longest row
     Blocked blk;             // LDS-staged layout (built for ray-traced matrices)
-    std::vector<std::unique_ptr<LevelSched>> lev;   // Kaczmarz level schedules, oldest first
+    // Kaczmarz level schedules, least recently used first; shared so that a caller holding one
+    // keeps it (and its graph and device buffers) alive when a later plan evicts it
+    std::vector<std::shared_ptr<LevelSched>> lev;
     std::vector<int32_t> hIA, hJA;                  // host copy of the pattern for level planning
                                                     // (kept up to 64 M entries)
 };
@@ -171,6 +173,10 @@ struct fair_csr {
     // Geometry of a ray-traced matrix (image side, rays per angle, angles); 0 when unknown.
     int32_t g_n = 0, g_p = 0, g_nth = 0;
     bool transposed = false;   // fair_transpose swapped the roles of A and T
+    // Some row holds a column twice (possible only in a caller-made matrix; a transpose keeps
+    // it).  Products sum such entries as the reference does; the Kaczmarz row updates, which
+    // update every entry of a row from the same x, refuse such a matrix.
+    bool dup_cols = false;
     bool blk_tuned = false;    // staged layouts chosen by fair_autotune_spmv (possibly none)
     // SIRT weights cached across calls (index: FAIR_CIMMINO / FAIR_SART) and loop workspace;
     // dropped whenever the matrix changes.
@@ -180,11 +186,9 @@ struct fair_csr {
     fair::DBuf<float> ws_r32, ws_x32, ws_rb, ws_xb;   // FP32 / batched workspaces
     bool graphs_used = false;
     int spmm_kc[2] = {16, 16};   // batched SpMM right-hand sides per wavefront (A, A^T)
-    void drop_cache() {
-        w_ok[0] = w_ok[1] = false;
-        A.vA32.release();
-        T.vA32.release();
-    }
+    // Forgets the weights and FP32 copies after the matrix changed, and releases the captured
+    // batched-SIRT graphs, which hold pointers to them (defined in sparse.hip).
+    void drop_cache();
 };
 
 namespace fair {
@@ -204,6 +208,11 @@ void ensure_blocked(fair_csr *h);
 // device CSR of A^T; no LDS-staged layout, no workspaces.
 void sirt_weights(fair_csr *h, int method);
 void drop_graphs(const fair_csr *h);
+// Refuses a matrix with a column held twice by one row (the Kaczmarz row updates).
+inline void check_unique_columns(const fair_csr *h) {
+    FAIR_CHECK(!h->dup_cols, FAIR_ERR_ARG,
+               "a row holds the same column twice: the Kaczmarz row updates need unique columns per row");
+}
 void ensure_T(fair_csr *h);
 void invalidate_T(fair_csr *h);
 

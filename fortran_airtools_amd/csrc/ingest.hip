@@ -21,7 +21,11 @@ __global__ void __launch_bounds__(INGEST_THREADS)
     }
 }
 
-int64_t kept_angles(int64_t nth, int64_t skip) { return skip > 0 ? (nth - 1) / skip + 1 : nth; }
+// Angles kept by the cull: 1, 1+skip, ... up to nth (none of none).
+int64_t kept_angles(int64_t nth, int64_t skip) {
+    if (nth <= 0) return 0;
+    return skip > 0 ? (nth - 1) / skip + 1 : nth;
+}
 
 void launch_absorbance(const double *T, double *b, int64_t p, int64_t nth, int64_t skip) {
     const int64_t keep = kept_angles(nth, skip);
@@ -48,7 +52,7 @@ using namespace fair;
 extern "C" int fair_absorbance_dev(const double *T, int64_t p, int64_t nth, int64_t skip, double *b) {
     return guard([&] {
         check_sizes(p, nth, skip);
-        FAIR_CHECK((T && b) || p * nth == 0, FAIR_ERR_ARG, "null argument");
+        FAIR_CHECK((T && b) || kept_angles(nth, skip) * p == 0, FAIR_ERR_ARG, "null argument");
         launch_absorbance(T, b, p, nth, skip);
     });
 }
@@ -57,7 +61,7 @@ extern "C" int fair_absorbance_dev(const double *T, int64_t p, int64_t nth, int6
 extern "C" int fair_absorbance(const double *T, int64_t p, int64_t nth, int64_t skip, double *b) {
     return guard([&] {
         check_sizes(p, nth, skip);
-        FAIR_CHECK((T && b) || p * nth == 0, FAIR_ERR_ARG, "null argument");
+        FAIR_CHECK((T && b) || kept_angles(nth, skip) * p == 0, FAIR_ERR_ARG, "null argument");
         const int64_t nb = kept_angles(nth, skip) * p;
         if (nb == 0) return;
         DBuf<double> dT((size_t)(p * nth)), db((size_t)nb);

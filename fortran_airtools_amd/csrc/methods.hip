@@ -641,13 +641,19 @@ int32_t levelize(int32_t n, const int32_t *IA, const int32_t *JA, int32_t base, 
 // cached in the matrix; nullptr when it is switched off (FAIR_ART_LEVELS=0) or the sequence has
 // fewer than FAIR_ART_LEVEL_WIDTH (4) steps per level on average, where one wavefront walking the
 // sequence is as fast.  A long sequence whose first eighth is already that narrow is rejected
-// before it is planned in full.
-const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
+// before it is planned in full.  The cache holds six schedules and evicts the least recently used
+// one; the returned reference keeps its schedule alive even if a later plan evicts it.
+std::shared_ptr<LevelSched> art_levels(Csr &A, const int32_t *seq, int64_t S) {
     if (S <= 0 || S > INT32_MAX || A.n <= 0 || !env_flag("FAIR_ART_LEVELS", 1)) return nullptr;
     const int64_t width = std::max(1, env_flag("FAIR_ART_LEVEL_WIDTH", 4));
-    for (auto &c : A.lev)
-        if ((int64_t)c->seq.size() == S && std::equal(c->seq.begin(), c->seq.end(), seq))
-            return S >= width * c->nlev ? c.get() : nullptr;
+    for (size_t i = 0; i < A.lev.size(); ++i) {
+        std::shared_ptr<LevelSched> c = A.lev[i];
+        if ((int64_t)c->seq.size() == S && std::equal(c->seq.begin(), c->seq.end(), seq)) {
+            A.lev.erase(A.lev.begin() + (ptrdiff_t)i);   // a hit becomes the most recently used
+            A.lev.push_back(c);
+            return S >= width * c->nlev ? c : nullptr;
+        }
+    }
     if (A.hIA.empty()) {
         A.hIA.resize((size_t)A.m + 1);
         A.hJA.resize((size_t)std::max<int64_t>(A.nnz, 1));
@@ -676,7 +682,7 @@ const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
         const int32_t Lp = levelize(A.n, IA, JA, 0, seq, P, ordered.data(), off.data());
         if (P < width * Lp) return nullptr;
     }
-    auto c = std::make_unique<LevelSched>();
+    auto c = std::make_shared<LevelSched>();
     c->seq.assign(seq, seq + S);
     const int32_t L = levelize(A.n, IA, JA, 0, seq, S, ordered.data(), off.data());
     off.resize((size_t)L + 1);
@@ -699,8 +705,8 @@ const LevelSched *art_levels(Csr &A, const int32_t *seq, int64_t S) {
     FAIR_HIP(hipStreamSynchronize(stream()));
     c->hoff = std::move(off);
     if (A.lev.size() >= 6) A.lev.erase(A.lev.begin());
-    A.lev.push_back(std::move(c));
-    return S >= width * L ? A.lev.back().get() : nullptr;
+    A.lev.push_back(c);
+    return S >= width * L ? c : nullptr;
 }
 
 // The schedule as one launch per level: captured into a graph on first use, replayed on the
@@ -746,7 +752,7 @@ hipEvent_t g_level_ev[2] = {nullptr, nullptr};
 
 // One level-parallel sweep per chain (s1 / x1 may be null): one launch per level per chain, or
 // with FAIR_ART_LEVEL_MODE=workgroup one workgroup per chain.
-void launch_art_levels(const Csr &A, const LevelSched *s0, double *x0, const LevelSched *s1, double *x1,
+void launch_art_levels(const Csr &A, LevelSched *s0, double *x0, LevelSched *s1, double *x1,
                        const double *b, const double *nrA, double relax, const Box &box) {
     const char *mode = getenv("FAIR_ART_LEVEL_MODE");
     if (!(mode && !strcmp(mode, "workgroup"))) {
@@ -754,8 +760,8 @@ void launch_art_levels(const Csr &A, const LevelSched *s0, double *x0, const Lev
             fprintf(stderr, "[fair] level sweep (graph): %zu steps in %d levels%s\n", s0->seq.size(), s0->nlev,
                     s1 ? " (two chains)" : "");
         if (!s1 || s1 == s0) {   // one chain, or two chains of one sequence: one after the other
-            graph_art_levels(A, const_cast<LevelSched *>(s0), x0, b, nrA, relax, box, stream());
-            if (s1) graph_art_levels(A, const_cast<LevelSched *>(s1), x1, b, nrA, relax, box, stream());
+            graph_art_levels(A, s0, x0, b, nrA, relax, box, stream());
+            if (s1) graph_art_levels(A, s1, x1, b, nrA, relax, box, stream());
             return;
         }
         // two independent chains (twin_art): the second replays on a side stream alongside the
@@ -766,8 +772,8 @@ void launch_art_levels(const Csr &A, const LevelSched *s0, double *x0, const Lev
         }
         FAIR_HIP(hipEventRecord(g_level_ev[0], stream()));
         FAIR_HIP(hipStreamWaitEvent(g_level_side, g_level_ev[0], 0));
-        graph_art_levels(A, const_cast<LevelSched *>(s1), x1, b, nrA, relax, box, g_level_side);
-        graph_art_levels(A, const_cast<LevelSched *>(s0), x0, b, nrA, relax, box, stream());
+        graph_art_levels(A, s1, x1, b, nrA, relax, box, g_level_side);
+        graph_art_levels(A, s0, x0, b, nrA, relax, box, stream());
         FAIR_HIP(hipEventRecord(g_level_ev[1], g_level_side));
         FAIR_HIP(hipStreamWaitEvent(stream(), g_level_ev[1], 0));
         return;
@@ -998,6 +1004,7 @@ ArtPlan plan_art(fair_csr *h, int method, int nk, const int32_t *Kin, const int3
                  const fair_opts *o, DBuf<double> &d_nrA) {
     FAIR_CHECK(method >= FAIR_KACZMARZ && method <= FAIR_RANDKACZMARZ, FAIR_ERR_METHOD,
                "Unknown ART method. Use kaczmarz, symkaczmarz or randkaczmarz");
+    check_unique_columns(h);
     FAIR_CHECK(nk >= 1 && Kin, FAIR_ERR_ARG, "Kin must not be empty");
     Csr &A = h->A;
     if (A.n_rows_nz < 0) build_row_list(A);
@@ -1120,6 +1127,7 @@ extern "C" int fair_art_steps(fair_csr *h, const int32_t *rows, int64_t nsteps, 
                               const fair_opts *o, double *x) {
     return guard([&] {
         check_handle(h);
+        check_unique_columns(h);
         FAIR_CHECK(x && b && nrA && (rows || nsteps == 0) && nsteps >= 0, FAIR_ERR_ARG, "null argument");
         const int32_t m = h->A.m, n = h->A.n;
         for (int64_t s = 0; s < nsteps; ++s)
@@ -1134,8 +1142,9 @@ extern "C" int fair_art_steps(fair_csr *h, const int32_t *rows, int64_t nsteps, 
         const Box box = make_box(o);
         const double relax = o ? o->relaxpar : 1.0;
         // a caller repeating one sequence (cyclic or symmetric sweeps) reuses its level schedule
-        if (const LevelSched *lev = art_levels(h->A, rows, nsteps)) {
-            launch_art_levels(h->A, lev, dx.p, nullptr, nullptr, db.p, dw.p, relax, box);
+        const std::shared_ptr<LevelSched> lev = art_levels(h->A, rows, nsteps);
+        if (lev) {
+            launch_art_levels(h->A, lev.get(), dx.p, nullptr, nullptr, db.p, dw.p, relax, box);
         } else {
             launch_art_sweep(h->A, 1,
                                SweepChains{{dr.p, nullptr}, {nsteps, 0}, {dx.p, nullptr}}, h->A.IA.p, h->A.JA.p,
@@ -1151,6 +1160,7 @@ extern "C" int fair_art_steps_dev(fair_csr *h, const int32_t *d_rows, int64_t ns
                                   const double *d_nrA, const fair_opts *o, double *d_x) {
     return guard([&] {
         check_handle(h);
+        check_unique_columns(h);
         FAIR_CHECK(nsteps >= 0 && (nsteps == 0 || (d_rows && d_b && d_nrA && d_x)), FAIR_ERR_ARG,
                    "null argument");
         const Box box = make_box(o);
@@ -1226,7 +1236,7 @@ extern "C" int fair_twin_art(fair_csr *h, int32_t method, int32_t stoprule, cons
             q += 2 * (int64_t)P.nord;
         };
         const unsigned nb = grid_for(n, 256);
-        const LevelSched *lx = nullptr, *ly = nullptr;
+        std::shared_ptr<LevelSched> lx, ly;   // held for the whole call
         bool planned = false;
         // both chains' sweeps of one iteration, draws taken in the reference's call order
         auto sweep_both = [&] {
@@ -1241,7 +1251,7 @@ extern "C" int fair_twin_art(fair_csr *h, int32_t method, int32_t stoprule, cons
             }
             planned = true;
             if (lx && ly) {
-                launch_art_levels(h->A, lx, x.p, ly, y.p, db.p, d_nrA.p, relax, box);
+                launch_art_levels(h->A, lx.get(), x.p, ly.get(), y.p, db.p, d_nrA.p, relax, box);
                 return;
             }
             dsx.upload(sx.data(), per_chain);
@@ -1386,7 +1396,7 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
     size_t rec = 0;
     if (exit_iter) *exit_iter = 0;
     const unsigned nb = grid_for(n, 256);
-    const LevelSched *lev = nullptr;
+    std::shared_ptr<LevelSched> lev;   // held for the whole call
     for (int iter = 1; iter <= P.maxiter; ++iter) {
         FAIR_CHECK(q + per_iter <= nu || !P.isrand, FAIR_ERR_ARG, "too few uniforms: need " + std::to_string(need));
         if (P.isrand) {                               // :320
@@ -1409,7 +1419,7 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
         // without draws every iteration sweeps the same sequence: one level schedule serves all
         if (iter == 1 && !P.isrand) lev = art_levels(h->A, seq.data(), steps_per_iter);
         if (lev) {
-            launch_art_levels(h->A, lev, dx.p, nullptr, nullptr, db.p, d_nrA.p, relax, box);
+            launch_art_levels(h->A, lev.get(), dx.p, nullptr, nullptr, db.p, d_nrA.p, relax, box);
         } else if (steps_per_iter > 0) {
             dseq.upload(seq.data(), steps_per_iter);
             launch_art_sweep(h->A, 1,

@@ -428,7 +428,21 @@ extern "C" int fair_csr_create(int32_t m, int32_t n, int64_t nnz, const int32_t 
             FAIR_CHECK(IA[r + 1] >= IA[r], FAIR_ERR_ARG, "IA must be non-decreasing");
         for (int64_t k = 0; k < nnz; ++k)
             FAIR_CHECK(JA[k] >= 1 && JA[k] <= n, FAIR_ERR_ARG, "column index out of range");
+        // a column held twice by one row: mark[c] is the last row (one-based) that held c
+        bool dup = false;
+        if (nnz > 0) {
+            std::vector<int32_t> mark((size_t)n + 1, 0);
+            for (int32_t r = 0; r < m && !dup; ++r)
+                for (int32_t k = IA[r]; k < IA[r + 1]; ++k) {
+                    if (mark[JA[k]] == r + 1) {
+                        dup = true;
+                        break;
+                    }
+                    mark[JA[k]] = r + 1;
+                }
+        }
         auto *h = new fair_csr();
+        h->dup_cols = dup;
         try {
             Csr &A = h->A;
             A.m = m;
@@ -498,6 +512,13 @@ extern "C" int fair_csr_device_arrays(const fair_csr *h, const int32_t **IA, con
     });
 }
 
+void fair_csr::drop_cache() {
+    w_ok[0] = w_ok[1] = false;
+    A.vA32.release();
+    T.vA32.release();
+    if (graphs_used) fair::drop_graphs(this);
+}
+
 extern "C" int fair_csr_free(fair_csr *h) {
     return guard([&] {
         if (h) {
@@ -563,6 +584,7 @@ extern "C" int fair_transvecmul(fair_csr *h, const double *v, int32_t nv, double
 extern "C" int fair_rowvecmult(fair_csr *h, int32_t k, const double *v, int32_t nv, double *f) {
     return guard([&] {
         check_handle(h);
+        check_unique_columns(h);
         FAIR_CHECK(k >= 1 && k <= h->A.m, FAIR_ERR_ARG, "row index out of range");
         FAIR_CHECK(nv == h->A.n, FAIR_ERR_ARG, "Vector input size incorrect.");
         DBuf<double> dv(nv > 0 ? nv : 1), df(1);
@@ -633,6 +655,7 @@ extern "C" int fair_add_scalar_mult_row(fair_csr *h, int32_t k, double c, double
                                         const double *lwrb, const double *upb) {
     return guard([&] {
         check_handle(h);
+        check_unique_columns(h);
         FAIR_CHECK(k >= 1 && k <= h->A.m, FAIR_ERR_ARG, "row index out of range");
         FAIR_CHECK(nw == h->A.n, FAIR_ERR_ARG, "Vector input size incorrect.");
         DBuf<double> dw(nw > 0 ? nw : 1);

@@ -178,3 +178,18 @@ def test_random_spellings_against_compiled_reference(seed):
     assert (dat.n, dat.p, dat.nth) == geom
     assert bitequal(np.array([dat.r, dat.dw, dat.sd]), np.array(rdwsd))
     assert bitequal(dat.theta, th_ref) and bitequal(dat.T, T_ref)
+
+
+def test_absorbance_size_arithmetic_without_device():
+    """fair_absorbance's size checks run before any device call: no angles with a cull step keeps
+    no angles (a null T is then fine), while kept angles with a null T are refused."""
+    import ctypes as C
+    from fortran_airtools_amd import _lib
+    L = _lib.load()
+    assert L.fair_absorbance(None, 5, 0, 2, None) == _lib.FAIR_OK
+    assert L.fair_absorbance(None, 5, 0, 0, None) == _lib.FAIR_OK
+    assert L.fair_absorbance(None, 0, 7, 3, None) == _lib.FAIR_OK
+    assert L.fair_absorbance(None, 5, 3, 2, None) == _lib.FAIR_ERR_ARG
+    assert L.fair_absorbance_dev(None, 5, 3, 2, None) == _lib.FAIR_ERR_ARG
+    assert L.fair_absorbance_dev(None, 5, 0, 2, None) == _lib.FAIR_OK
+    assert L.fair_absorbance(None, -1, 3, 2, None) == _lib.FAIR_ERR_ARG

@@ -251,3 +251,51 @@ def test_schedule_cache_eviction(monkeypatch, capfd):
         lev, ref, used = both_paths(monkeypatch, capfd, run)
         assert used
         np.testing.assert_array_equal(lev, ref)
+
+
+def test_twin_art_after_cache_eviction(monkeypatch, capfd):
+    """A schedule in use stays alive while a later plan of the same call evicts entries: art plans
+    the order's sequence, five basic_sweep calls fill the six-entry cache behind it, then twin_art
+    with that order hits the cached forward sequence and plans the reversed one (a miss with the
+    cache full).  The forward schedule must survive, and the twin result equals the one-wavefront
+    sweep's bit for bit."""
+    _, _, am = fair()
+    A = fan(80, 121, 72)
+    b = rhs(A, 21)
+    nrA = A.row_norms(2) ** 2
+    rows = np.nonzero(nrA > 0)[0].astype(np.int32) + 1
+    w = np.where(nrA > 0, nrA, 1.0)
+    order = np.random.default_rng(22).permutation(rows)[: len(rows) // 2].astype(np.int32)
+    x0 = np.random.default_rng(23).random(A.n)
+    op = opts(am, relax=1.0, lb=0.0, ub=1.0)
+
+    def run():
+        am.art("kaczmarz", A, b, [1], order=order, options=op)
+        for k in range(5):
+            am.basic_sweep(x0, A, b, np.roll(rows, 53 * (k + 1)), len(rows), False, False, 1.0, w, None, True, True,
+                           1.0, 0.0)
+        return am.twin_art("kaczmarz", "errorgauge", A, b, [2, 4], order=order, options=op)
+
+    lev, seq, used = both_paths(monkeypatch, capfd, run)
+    assert used
+    for a, c in zip(lev, seq):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(c))
+
+
+def test_duplicate_columns_refused_by_kaczmarz():
+    """A caller-made matrix whose row holds a column twice: products sum the entries as the
+    reference's loops do, while the Kaczmarz paths (whose lanes update every entry of a row from
+    the same x) refuse it with an error instead of a silently different iterate."""
+    from fortran_airtools_amd._lib import FairError
+    _, sm, am = fair()
+    IA = np.array([0, 3, 5], np.int32)
+    JA = np.array([1, 3, 1, 2, 3], np.int32)
+    vA = np.array([1.0, 2.0, 0.5, 4.0, 1.5])
+    A = sm.CSRMatrix.from_arrays(2, 3, IA, JA, vA)
+    x = np.array([1.0, 2.0, 3.0])
+    np.testing.assert_array_equal(A.matvecmul(x), [1.0 + 2.0 * 3.0 + 0.5, 4.0 * 2.0 + 1.5 * 3.0])
+    b = np.array([1.0, 1.0])
+    with pytest.raises(FairError, match="same column twice"):
+        am.art("kaczmarz", A, b, [1])
+    with pytest.raises(FairError, match="same column twice"):
+        am.twin_art("kaczmarz", "errorgauge", A, b, [2])
