@@ -125,6 +125,34 @@ def sirt_batch(sirt_method: str, A: CSRMatrix, B, Kin, X0=None, options: Opts | 
     return out
 
 
+def sirt_batch_dev(sirt_method: str, A: CSRMatrix, d_B: int, k: int, maxits: int, d_X: int,
+                   options: Opts | None = None, use_graph: bool = False) -> None:
+    """Batched FP32 sirt on HBM panels: d_B (m x k) and d_X (n x k, start in, result out), both
+    row-major FP32.  With use_graph the run is captured once and replayed on later calls with
+    the same arguments."""
+    if sirt_method not in SIRT_METHODS:
+        raise FairError(FAIR_ERR_METHOD, "Not a recognised SIRT method.")
+    o = _opts(options)
+    check(lib().fair_sirt_batch_dev(A.handle, SIRT_METHODS[sirt_method], C.c_void_p(d_B), int(k), int(maxits),
+                                    C.c_void_p(d_X), C.byref(o), int(bool(use_graph))))
+
+
+def autotune_batch(A: CSRMatrix, k: int) -> tuple:
+    """Times the batched SpMM's row orders with k right-hand sides and keeps the fastest for A*X
+    and A^T*R; returns the two order codes."""
+    out = (C.c_int32 * 2)()
+    check(lib().fair_autotune_batch(A.handle, int(k), out))
+    return out[0], out[1]
+
+
+def batch_relerr_dev(d_X: int, d_Xref: int, n: int, k: int) -> np.ndarray:
+    """Per column ||X_j - Xref_j|| / ||Xref_j|| of an FP32 panel X against an FP64 panel Xref
+    (both n x k row-major on the device), with FP64 sums."""
+    err = np.empty(int(k))
+    check(lib().fair_batch_relerr_dev(C.c_void_p(d_X), C.c_void_p(d_Xref), int(n), int(k), _ptr(err)))
+    return err
+
+
 def normest(A: CSRMatrix, tolerance: float = 1e-5, max_its: int = 100, v0=None, seed: int = 0) -> float:
     """normest (AIRMs.f90:1129-1173): power iteration on A^T A from v0 (uniform [0,1) draws,
     seeded here when not supplied)."""

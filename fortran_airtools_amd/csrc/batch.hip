@@ -100,75 +100,129 @@ __global__ void k_copy_f32(const float *a, float *b, int64_t n) {
 void check_handle(const fair_csr *A) { FAIR_CHECK(A != nullptr, FAIR_ERR_ARG, "null matrix handle"); }
 
 // ---- batched: SpMM ----------------------------------------------------------------------------
-// KC right-hand sides per wavefront; 64/KC matrix entries side by side.  Blocks are numbered
-// phase-major: consecutive blocks b, b+1, ... go to XCDs b % 8, and phase ph covers chunks
-// 8*ph .. 8*ph+7 with chunk 8*ph + (b % 8) on XCD b % 8, so while a phase runs every XCD
-// streams the matrix against one chunk of X whose slice (n * KC floats) stays in its L2.
-constexpr int SPMM_UNROLL = 4;
+// One wavefront per matrix row.  Each lane holds VEC consecutive right-hand sides (VEC = 4 when k
+// is a multiple of 4), so one pass covers 64*VEC of them (all 256 of a k = 256 batch) and every
+// matrix entry gathers one contiguous, coalesced row segment of X (1 KB at k = 256): the matrix
+// is read once per product.  A row's entries are loaded 64 at a time, one per lane, and
+// broadcast with v_readlane; SPMM_U gathers are issued before their products are summed.
+// Columns outside [0, ncols) and the lanes past k read a zero row instead (their weight is 0).
+//
+// Rows run in a tuned order (spmm_order below): rays grouped by sinogram tiles (angles x
+// detectors) for A, pixels by image tiles for A^T, so that the rows in flight on one XCD share
+// their footprint in its L2.  The hardware deals block b to XCD b % 8; logical block
+// (b % 8) * per_xcd + b / 8 gives every XCD a contiguous stretch of the order.
 constexpr int NXCD = 8;
+constexpr int SPMM_U = 8;
+constexpr int SPMM_WAVES = 4;   // rows (wavefronts) per 256-thread block
 
-template <int KC, class Epi>
-__global__ void __launch_bounds__(256) k_spmm(const int4 *__restrict__ rows, int32_t nrows,
-                                              const int32_t *__restrict__ JA,
-                                              const float *__restrict__ vA, int32_t ncols,
-                                              const float *__restrict__ X, int32_t k, int32_t nchunks,
-                                              int32_t nrb, Epi epi) {
-    constexpr int SLOTS = WAVE / KC;
+template <int V>
+struct FV;
+template <>
+struct FV<1> {
+    float v[1];
+    __device__ __forceinline__ static FV load(const float *p) { return FV{{*p}}; }
+};
+template <>
+struct FV<4> {
+    float v[4];
+    __device__ __forceinline__ static FV load(const float *p) {
+        const float4 t = *reinterpret_cast<const float4 *>(p);
+        return FV{{t.x, t.y, t.z, t.w}};
+    }
+};
+template <int V>
+__device__ __forceinline__ void store_fv(float *p, const float (&o)[V]) {
+    if constexpr (V == 4) *reinterpret_cast<float4 *>(p) = make_float4(o[0], o[1], o[2], o[3]);
+    else p[0] = o[0];
+}
+
+template <int VEC, class Epi>
+__global__ void __launch_bounds__(256) k_spmm_rows(const int4 *__restrict__ rows, int32_t nrows, int32_t per_xcd,
+                                                   const int32_t *__restrict__ JA, const float *__restrict__ vA,
+                                                   int32_t ncols, const float *__restrict__ X, int32_t k,
+                                                   const float *__restrict__ zrow, Epi epi) {
     const int b = blockIdx.x;
-    const int phase = b / (NXCD * nrb);
-    const int chunk = phase * NXCD + (b % NXCD);
-    if (chunk >= nchunks) return;
-    const int rb = (b / NXCD) % nrb;
-    const int wv = rb * 4 + (threadIdx.x >> 6);
+    const int lb = (b % NXCD) * per_xcd + b / NXCD;
+    const int wv = __builtin_amdgcn_readfirstlane(lb * SPMM_WAVES + (int)(threadIdx.x >> 6));
     if (wv >= nrows) return;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int slot = lane / KC, j = lane % KC;
-    const int col0 = chunk * KC + j;
-    const bool active = col0 < k;
     const int4 ri = rows[wv];
-    float acc = 0.f;
-    for (int base = ri.y + slot; base < ri.z; base += SLOTS * SPMM_UNROLL) {
-        int32_t c[SPMM_UNROLL];
-        float v[SPMM_UNROLL];
+    for (int c0 = 0; c0 < k; c0 += WAVE * VEC) {
+        const int col = c0 + lane * VEC;
+        const bool lane_in = col < k;
+        // lanes past k read the start of the gathered row (their sums are dropped); masked
+        // entries read the zero row at the same lane offset
+        const uint32_t offb = 4u * (uint32_t)(lane_in ? col : c0);   // byte offset in a gathered row
+        const float *zbase = zrow - c0;
+        float acc[VEC];
 #pragma unroll
-        for (int u = 0; u < SPMM_UNROLL; ++u) {
-            const int q = base + SLOTS * u;
-            c[u] = q < ri.z ? JA[q] : -1;
-            v[u] = q < ri.z ? vA[q] : 0.f;
+        for (int e = 0; e < VEC; ++e) acc[e] = 0.f;
+        for (int base = ri.y; base < ri.z; base += WAVE) {
+            const int q = base + lane;
+            int cj = q < ri.z ? JA[q] : -1;
+            float vj = q < ri.z ? vA[q] : 0.f;
+            if ((uint32_t)cj >= (uint32_t)ncols) {
+                cj = -1;
+                vj = 0.f;
+            }
+            const int cnt = min(WAVE, ri.z - base);
+            for (int t = 0; t < cnt; t += SPMM_U) {
+                FV<VEC> xv[SPMM_U];
+                float w[SPMM_U];
+#pragma unroll
+                for (int u = 0; u < SPMM_U; ++u) {
+                    const int c = __builtin_amdgcn_readlane(cj, t + u);
+                    w[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vj), t + u));
+                    const float *rowp = X + (int64_t)max(c, 0) * k;
+                    rowp = c >= 0 ? rowp : zbase;
+                    xv[u] = FV<VEC>::load(reinterpret_cast<const float *>(reinterpret_cast<const char *>(rowp) + offb));
+                }
+#pragma unroll
+                for (int u = 0; u < SPMM_U; ++u)
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) acc[e] += w[u] * xv[u].v[e];
+            }
         }
-#pragma unroll
-        for (int u = 0; u < SPMM_UNROLL; ++u)
-            if ((uint32_t)c[u] < (uint32_t)ncols && active) acc += v[u] * X[(int64_t)c[u] * k + col0];
+        if (lane_in) epi.template apply<VEC>(ri.x, col, acc);
     }
-#pragma unroll
-    for (int off = KC; off < WAVE; off <<= 1) acc += __shfl_xor(acc, off, WAVE);
-    if (slot == 0 && active) epi(ri.x, col0, acc);
 }
 
 template <class Epi>
 __global__ void k_spmm_empty(const int32_t *__restrict__ rows, int32_t nrows, int32_t k, Epi epi) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nrows * k) return;
-    epi(rows[t / k], (int)(t % k), 0.f);
+    const float z[1] = {0.f};
+    epi.template apply<1>(rows[t / k], (int)(t % k), z);
 }
 
 struct EpiStoreB {
     float *Y;
     int32_t k;
-    __device__ __forceinline__ void operator()(int32_t i, int j, float acc) const { Y[(int64_t)i * k + j] = acc; }
+    template <int V>
+    __device__ __forceinline__ void apply(int32_t i, int j, const float (&acc)[V]) const {
+        store_fv<V>(Y + (int64_t)i * k + j, acc);
+    }
 };
 
+// r = (b - Ax) / M, FP64 scaling
 struct EpiResidualB {
     const float *B;
     const double *M;
     float *R;
     int32_t k;
-    __device__ __forceinline__ void operator()(int32_t i, int j, float acc) const {
+    template <int V>
+    __device__ __forceinline__ void apply(int32_t i, int j, const float (&acc)[V]) const {
         const int64_t q = (int64_t)i * k + j;
-        R[q] = (float)(((double)B[q] - (double)acc) / M[i]);
+        const double mi = M[i];
+        const FV<V> bv = FV<V>::load(B + q);
+        float o[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) o[e] = (float)(((double)bv.v[e] - (double)acc[e]) / mi);
+        store_fv<V>(R + q, o);
     }
 };
 
+// x_new = box(x + relax * (A^T r) / D), FP64 scaling
 struct EpiUpdateB {
     const float *X;
     const double *D;
@@ -176,43 +230,102 @@ struct EpiUpdateB {
     int32_t k;
     double relax;
     BoxF box;
-    __device__ __forceinline__ void operator()(int32_t p, int j, float acc) const {
+    template <int V>
+    __device__ __forceinline__ void apply(int32_t p, int j, const float (&acc)[V]) const {
         const int64_t q = (int64_t)p * k + j;
-        double u = relax * (double)acc;
-        if (D) u = u / D[p];
-        Xn[q] = box.apply((float)((double)X[q] + u));
+        const double dp = D ? D[p] : 1.0;
+        const FV<V> xv = FV<V>::load(X + q);
+        float o[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            double u = relax * (double)acc[e];
+            if (D) u = u / dp;
+            o[e] = box.apply((float)((double)xv.v[e] + u));
+        }
+        store_fv<V>(Xn + q, o);
     }
 };
 
-template <int KC, class Epi>
-void spmm_launch_kc(const Csr &A, const float *X, int32_t k, Epi epi, hipStream_t s) {
-    const int nchunks = (k + KC - 1) / KC;
-    const int nrb = (A.n_rows_nz + 3) / 4;
-    const int phases = (nchunks + NXCD - 1) / NXCD;
-    const int64_t blocks = (int64_t)phases * NXCD * nrb;
-    hipLaunchKernelGGL((k_spmm<KC, Epi>), dim3((unsigned)blocks), dim3(256), 0, s, (const int4 *)A.rows_nz.p,
-                       A.n_rows_nz, A.JA.p, A.vA32.p, A.n, X, k, nchunks, nrb, epi);
+__device__ float g_zero_row[WAVE * 4];   // zero-initialised: the gather target of masked entries
+
+const float *zero_row() {
+    static const float *p = nullptr;
+    if (!p) {
+        void *q = nullptr;
+        FAIR_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(g_zero_row)));
+        p = static_cast<const float *>(q);
+    }
+    return p;
 }
 
-// Right-hand sides per wavefront: 8, 16 or 32 (chosen by fair_autotune_batch, default 16).
+// Row orders of the SpMM.  Code 0 keeps the compacted row list's order; codes 1.. group rays
+// (rows of a ray-traced A: angle a = r / p, detector d = r % p) into BA x BD sinogram tiles or
+// pixels (rows of A^T: image column x = j / N, row y = j % N) into TX x TY image tiles, tiles in
+// row-major order, rows inside a tile likewise.
+constexpr int SPMM_ORDERS = 8;
+constexpr int RAY_TILES[SPMM_ORDERS][2] = {{0, 0}, {16, 16}, {32, 8}, {64, 4}, {64, 8}, {128, 4}, {32, 16}, {128, 2}};
+constexpr int PIX_TILES[SPMM_ORDERS][2] = {{0, 0}, {8, 8}, {16, 16}, {32, 32}, {16, 32}, {32, 16}, {64, 64}, {16, 8}};
+
+// The row descriptors of M in order `code` (built once per code and cached in M); rays says
+// whether M's rows are rays (A of a ray-traced matrix) or pixels (its transpose).
+const int4 *spmm_order(const fair_csr *h, Csr &M, bool rays, int code) {
+    const bool geo = h->g_n > 0 && h->g_p > 0 && h->g_nth > 0;
+    if (code <= 0 || code >= SPMM_ORDERS || !geo || M.n_rows_nz <= 0) return (const int4 *)M.rows_nz.p;
+    if (M.spmm_code == code && M.spmm_rows.p) return (const int4 *)M.spmm_rows.p;
+    const int64_t nr = M.n_rows_nz;
+    std::vector<int4> d((size_t)nr);
+    FAIR_HIP(hipMemcpyAsync(d.data(), M.rows_nz.p, sizeof(int4) * nr, hipMemcpyDeviceToHost, stream()));
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    const int64_t N = h->g_n, P = h->g_p;
+    const int t0 = rays ? RAY_TILES[code][0] : PIX_TILES[code][0];
+    const int t1 = rays ? RAY_TILES[code][1] : PIX_TILES[code][1];
+    const int64_t n1 = rays ? (P + t1 - 1) / t1 : (N + t1 - 1) / t1;
+    std::vector<std::pair<int64_t, int32_t>> key((size_t)nr);
+    for (int64_t s = 0; s < nr; ++s) {
+        const int64_t r = d[s].x;
+        const int64_t u = rays ? r / P : r / N, v = rays ? r % P : r % N;
+        key[s] = {((u / t0) * n1 + v / t1) * (int64_t)(t0 * t1) + (u % t0) * t1 + v % t1, (int32_t)s};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int4> o((size_t)nr);
+    for (int64_t s = 0; s < nr; ++s) o[s] = d[key[s].second];
+    M.spmm_rows.alloc(4 * (size_t)nr);
+    FAIR_HIP(hipMemcpyAsync(M.spmm_rows.p, o.data(), sizeof(int4) * nr, hipMemcpyHostToDevice, stream()));
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    M.spmm_code = code;
+    return (const int4 *)M.spmm_rows.p;
+}
+
+template <int VEC, class Epi>
+void spmm_launch_v(const int4 *rows, int32_t nrows, const Csr &M, const float *X, int32_t k, Epi epi,
+                   hipStream_t s) {
+    const int64_t blocks = (nrows + SPMM_WAVES - 1) / SPMM_WAVES;
+    const int64_t per_xcd = (blocks + NXCD - 1) / NXCD;
+    hipLaunchKernelGGL((k_spmm_rows<VEC, Epi>), dim3((unsigned)(per_xcd * NXCD)), dim3(256), 0, s, rows, nrows,
+                       (int32_t)per_xcd, M.JA.p, M.vA32.p, M.n, X, k, zero_row(), epi);
+}
+
+// Y = M * X (FP32) through Epi, rows of M in order `code`.
 template <class Epi>
-void spmm_launch(const Csr &A, const float *X, int32_t k, int kc, Epi epi, hipStream_t s) {
-    if (A.n_rows_nz <= 0 || k <= 0) return;
-    if (kc == 8) spmm_launch_kc<8>(A, X, k, epi, s);
-    else if (kc == 32) spmm_launch_kc<32>(A, X, k, epi, s);
-    else spmm_launch_kc<16>(A, X, k, epi, s);
+void spmm_launch(const fair_csr *h, Csr &M, bool rays, const float *X, int32_t k, int code, Epi epi,
+                 hipStream_t s) {
+    if (M.n_rows_nz <= 0 || k <= 0) return;
+    const int4 *rows = spmm_order(h, M, rays, code);
+    if (k % 4 == 0) spmm_launch_v<4>(rows, M.n_rows_nz, M, X, k, epi, s);
+    else spmm_launch_v<1>(rows, M.n_rows_nz, M, X, k, epi, s);
 }
 
 // Cached HIP graphs of whole batched runs, keyed by the run's arguments.
 // A captured run also bakes in handle-owned pointers (weights, FP32 values, layouts, workspaces):
-// whatever replaces them drops the handle's graphs.  The tuned widths are part of the key.
+// whatever replaces them drops the handle's graphs.  The tuned row orders are part of the key.
 struct GraphKey {
     const void *h, *B, *X;
-    int32_t method, k, maxits, upb, lwrb, kc0, kc1;
+    int32_t method, k, maxits, upb, lwrb, ord0, ord1;
     double relax, ub, lb;
     bool operator<(const GraphKey &o) const {
-        return std::tie(h, B, X, method, k, maxits, upb, lwrb, kc0, kc1, relax, ub, lb) <
-               std::tie(o.h, o.B, o.X, o.method, o.k, o.maxits, o.upb, o.lwrb, o.kc0, o.kc1, o.relax, o.ub, o.lb);
+        return std::tie(h, B, X, method, k, maxits, upb, lwrb, ord0, ord1, relax, ub, lb) <
+               std::tie(o.h, o.B, o.X, o.method, o.k, o.maxits, o.upb, o.lwrb, o.ord0, o.ord1, o.relax, o.ub,
+                        o.lb);
     }
 };
 std::map<GraphKey, hipGraphExec_t> g_graphs;
@@ -227,17 +340,21 @@ namespace {
 
 void batch_iterations(fair_csr *h, int method, const float *dB, int32_t k, int32_t maxits, float *dX,
                       float *dR, float *dXn, const fair_opts *o, hipStream_t s) {
-    const Csr &A = h->A, &T = h->T;
+    Csr &A = h->A, &T = h->T;
+    const bool a_rays = !h->transposed;   // A's rows are the rays unless the caller transposed it
     const double *M = h->M[method].p;
     const double *D = method == FAIR_SART ? h->D.p : nullptr;
     const double relax = o ? o->relaxpar : 1.0;
     const BoxF box = make_boxf(o);
     float *cur = dX, *nxt = dXn;
-    const int kca = h->spmm_kc[0], kct = h->spmm_kc[1];
+    const int oa = h->spmm_tile[0], ot = h->spmm_tile[1];
+    // order the rows before a capture starts (the orders are built with synchronous copies)
+    spmm_order(h, A, a_rays, oa);
+    spmm_order(h, T, !a_rays, ot);
     for (int it = 0; it < maxits; ++it) {
-        spmm_launch(A, cur, k, kca, EpiResidualB{dB, M, dR, k}, s);
+        spmm_launch(h, A, a_rays, cur, k, oa, EpiResidualB{dB, M, dR, k}, s);
         EpiUpdateB up{cur, D, nxt, k, relax, box};
-        spmm_launch(T, dR, k, kct, up, s);
+        spmm_launch(h, T, !a_rays, dR, k, ot, up, s);
         if (T.n_rows_empty > 0) {
             const int64_t t = (int64_t)T.n_rows_empty * k;
             hipLaunchKernelGGL((k_spmm_empty<EpiUpdateB>), dim3(grid_for(t, 256)), dim3(256), 0, s,
@@ -332,7 +449,7 @@ extern "C" int fair_sirt_batch_dev(fair_csr *h, int32_t method, const float *d_B
         // graph path: capture the whole run once on a private stream, replay afterwards
         if (!g_capture) FAIR_HIP(hipStreamCreateWithFlags(&g_capture, hipStreamNonBlocking));
         GraphKey key{h,        d_B,          d_X,          method,       k,
-                     maxits,   o ? o->upb : 0, o ? o->lwrb : 0, h->spmm_kc[0], h->spmm_kc[1],
+                     maxits,   o ? o->upb : 0, o ? o->lwrb : 0, h->spmm_tile[0], h->spmm_tile[1],
                      o ? o->relaxpar : 1.0, o ? o->ubound : 0.0, o ? o->lbound : 1.0};
         auto it = g_graphs.find(key);
         if (it == g_graphs.end()) {
@@ -407,31 +524,35 @@ extern "C" int fair_batch_relerr_dev(const float *d_X, const double *d_Xref, int
     });
 }
 
-// Autotuner of the batched SpMM: times each candidate width on this matrix and k with HIP
-// events (one A*X and one A^T*R launch each, after a warm-up) and keeps the fastest per
+// Autotuner of the batched SpMM: times every row order (sinogram tiles for A, image tiles for
+// A^T; only the stored order for a matrix without geometry) on this matrix with k right-hand
+// sides with HIP events (two launches each, after a warm-up) and keeps the fastest per
 // direction in the handle.
-extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *kc_out) {
+extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
     return guard([&] {
         check_handle(h);
         FAIR_CHECK(k >= 1, FAIR_ERR_ARG, "k must be >= 1");
         sirt_setup_public(h, FAIR_SART);
         make_f32(h->A);
         make_f32(h->T);
-        const Csr &A = h->A, &T = h->T;
+        Csr &A = h->A, &T = h->T;
+        const bool a_rays = !h->transposed;
+        const bool geo = h->g_n > 0 && h->g_p > 0 && h->g_nth > 0;
         DBuf<float> X((size_t)A.n * k), R((size_t)A.m * k), Y((size_t)std::max(A.m, A.n) * k);
         FAIR_HIP(hipMemsetAsync(X.p, 0, X.bytes(), stream()));
         FAIR_HIP(hipMemsetAsync(R.p, 0, R.bytes(), stream()));
         hipEvent_t e0, e1;
         FAIR_HIP(hipEventCreate(&e0));
         FAIR_HIP(hipEventCreate(&e1));
-        const int cands[3] = {8, 16, 32};
+        const bool dbg = getenv("FAIR_DEBUG") && atoi(getenv("FAIR_DEBUG")) != 0;
         for (int dir = 0; dir < 2; ++dir) {
-            const Csr &M = dir == 0 ? A : T;
+            Csr &M = dir == 0 ? A : T;
+            const bool rays = dir == 0 ? a_rays : !a_rays;
             const float *src = dir == 0 ? X.p : R.p;
             float best = 1e30f;
-            int best_kc = 16;
-            for (int kc : cands) {
-                auto run = [&] { spmm_launch(M, src, k, kc, EpiStoreB{Y.p, k}, stream()); };
+            int best_code = 0;
+            for (int code = 0; code < (geo ? SPMM_ORDERS : 1); ++code) {
+                auto run = [&] { spmm_launch(h, M, rays, src, k, code, EpiStoreB{Y.p, k}, stream()); };
                 run();
                 FAIR_HIP(hipEventRecord(e0, stream()));
                 run();
@@ -440,19 +561,24 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *kc_out) {
                 FAIR_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
                 FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (dbg)
+                    fprintf(stderr, "[fair] spmm %s order %d (%dx%d): %.1f us\n", dir == 0 ? "A" : "A^T", code,
+                            rays ? RAY_TILES[code][0] : PIX_TILES[code][0],
+                            rays ? RAY_TILES[code][1] : PIX_TILES[code][1], ms * 500.f);
                 if (ms < best) {
                     best = ms;
-                    best_kc = kc;
+                    best_code = code;
                 }
             }
-            h->spmm_kc[dir] = best_kc;
+            h->spmm_tile[dir] = best_code;
+            spmm_order(h, M, rays, best_code);
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         launch_check("autotune batch");
-        if (kc_out) {
-            kc_out[0] = h->spmm_kc[0];
-            kc_out[1] = h->spmm_kc[1];
+        if (order_out) {
+            order_out[0] = h->spmm_tile[0];
+            order_out[1] = h->spmm_tile[1];
         }
     });
 }

@@ -154,6 +154,8 @@ struct Csr {
     int32_t n_rows_empty = 0;
     int32_t max_row = 0;     // longest row
     Blocked blk;             // LDS-staged layout (built for ray-traced matrices)
+    DBuf<int32_t> spmm_rows; // rows_nz in the batched SpMM's tile order spmm_code (int4 each)
+    int32_t spmm_code = -1;
     // Kaczmarz level schedules, least recently used first; shared so that a caller holding one
     // keeps it (and its graph and device buffers) alive when a later plan evicts it
     std::vector<std::shared_ptr<LevelSched>> lev;
@@ -185,7 +187,8 @@ struct fair_csr {
     fair::DBuf<double> ws_r, ws_x;
     fair::DBuf<float> ws_r32, ws_x32, ws_rb, ws_xb;   // FP32 / batched workspaces
     bool graphs_used = false;
-    int spmm_kc[2] = {16, 16};   // batched SpMM right-hand sides per wavefront (A, A^T)
+    // batched SpMM row orders (A, A^T): 0 = stored order, else a sinogram / image tile shape
+    int spmm_tile[2] = {2, 2};
     // Forgets the weights and FP32 copies after the matrix changed, and releases the captured
     // batched-SIRT graphs, which hold pointers to them (defined in sparse.hip).
     void drop_cache();

@@ -283,6 +283,8 @@ void build_row_list(Csr &A) {
     A.n_rows_nz = (int32_t)cnt;
     A.n_rows_empty = (int32_t)(m - cnt);
     A.max_row = hmx;
+    A.spmm_rows.release();   // the batched SpMM's reordered copy of the list
+    A.spmm_code = -1;
 }
 
 // Sort the entries of every listed row by column index (values follow).

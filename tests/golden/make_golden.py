@@ -13,6 +13,12 @@ Outputs (numpy .npz, inputs and expected outputs only):
   art_small.npz      Kaczmarz family on a small fan problem, with the uniform streams drawn
   c2.npz             (--c2) the N=512 problem: matrix hashes + sampled rows, the hash of b, SART
                      and Cimmino 200-iteration iterates
+  c3.npz             (--c3) the batched N=256 problem (256 slices, 360 x 363 fan): matrix hashes +
+                     sampled rows; for four slices the hash of b and the 300-iteration SART iterate
+  c5.npz             (--c5) the uneven batch: 20-iteration SART iterates of its three smallest
+                     problems (matrix and b hashes); randomized Kaczmarz with row-norm sampling and
+                     symmetric Kaczmarz, 20 sweeps on the N=256 fan, with the runtime draws' count and
+                     hash
   twin_small.npz     (--twin) twin_art with both stopping rules on a small fan problem, with the
                      uniform streams drawn
   dump_small.bin     (--dump) store_matrix's file of a small fan matrix (with dump_small.npz:
@@ -35,7 +41,7 @@ Outputs (numpy .npz, inputs and expected outputs only):
                      both seed-point types) and add_white_noise, each after seeding the runtime
                      generator with put(seed + 7919*i); the first draws of one seed
 
-usage: python tests/golden/make_golden.py [--c2 | --twin | --dump | --data | --artoracle | --sweep |
+usage: python tests/golden/make_golden.py [--c2 | --c3 | --c5 | --twin | --dump | --data | --artoracle | --sweep |
                                            --phantoms]
 """
 from __future__ import annotations
@@ -436,12 +442,99 @@ def write_c2(ref):
     np.savez_compressed(os.path.join(HERE, "c2.npz"), **d)
 
 
+C3_SLICES = (0, 1, 2, 129)   # one slice of each phantom kind (3phases, bubbles, smooth) and a late 3phases
+
+
+def c3_inputs(j: int) -> tuple:
+    """Slice j of the 256-slice batch: its phantom and noise seed (bench.py --mode batch)."""
+    gens = (cf.threephases, cf.bubbles, cf.smooth)
+    return gens[j % 3](cf.C3.n, seed=3 + j), 3 + j
+
+
+def write_c3(ref):
+    """The batched problem's shared matrix (hashes + sampled rows) and, for C3_SLICES, the
+    reference's FP64 SART after 300 iterations (box [0,1]) on each slice's noisy data; b of every
+    slice is rebuilt by the tests from the oracle's matvec and the seeded noise, checked by hash."""
+    g = cf.C3
+    th = g.theta()
+    IA, JA, vA = build(ref, 0, g.beam, g.n, g.p, g.nth, g.r, g.dw, g.sd, th)
+    d = matrix_summary("", IA, JA, vA, 1009)
+    d["slices"] = np.array(C3_SLICES, np.int32)
+    d["iterations"] = np.array(300)
+    for j in C3_SLICES:
+        xt, seed = c3_inputs(j)
+        ref.set(0, IA, JA, vA, g.ncols)
+        b = cf.add_noise(ref.matvec(0, xt), 0.01, seed)
+        d[f"slice{j}/b_sha"] = np.array(sha(b))
+        d[f"slice{j}/xt_sha"] = np.array(sha(xt))
+        d[f"slice{j}/sart_300"] = ref.sirt(0, "sart", b, 300, lb=0.0, ub=1.0, relax=1.0)
+        print("C3 slice", j, flush=True)
+    np.savez_compressed(os.path.join(HERE, "c3.npz"), **d)
+
+
+C5_KACZ_SEED = 5
+
+
+def c5_group_inputs(k: int, g: cf.Geometry) -> tuple:
+    """Problem k of the uneven group: its phantom and noise seed (bench.py --mode uneven)."""
+    gens = (cf.shepplogan, cf.bubbles, cf.smooth, cf.threephases)
+    x = cf.shepplogan(g.n) if k % 4 == 0 else gens[k % 4](g.n, seed=5 + k)
+    return x, 5 + k
+
+
+def write_c5(ref):
+    """The uneven batch and Kaczmarz problems.
+    Group: the reference's FP64 SART after 20 iterations (box [0,1]) for the three smallest
+    problems of configs.c5_problems(64, 5) (b checked by hash, as above).
+    Kaczmarz, on the N=256 fan of 360 x 363 rays (three-phase phantom seed 5, 1 % noise seed 5):
+    randomized Kaczmarz with row-norm sampling and symmetric Kaczmarz, 20 sweeps, box [0,1],
+    each after seeding the runtime generator with put(seed + 7919*i); the draws of the random run
+    are recorded by count and hash (the tests replay them from the same runtime)."""
+    probs = cf.c5_problems(64, 5)
+    small = sorted(range(len(probs)), key=lambda k: (probs[k].n * probs[k].n * probs[k].nth, k))[:3]
+    d = {"group_problems": np.array(small, np.int32), "group_iterations": np.array(20)}
+    for k in small:
+        g = probs[k]
+        IA, JA, vA = build(ref, 0, g.beam, g.n, g.p, g.nth, g.r, g.dw, g.sd, g.theta())
+        x, seed = c5_group_inputs(k, g)
+        b = cf.add_noise(ref.matvec(0, x), 0.01, seed)
+        d[f"p{k}/geom"] = np.array([g.n, g.p, g.nth], np.int64)
+        d[f"p{k}/IA_sha"], d[f"p{k}/JA_sha"], d[f"p{k}/vA_sha"] = (np.array(sha(IA)), np.array(sha(JA)),
+                                                                   np.array(sha(vA)))
+        d[f"p{k}/b_sha"] = np.array(sha(b))
+        ref.set(0, IA, JA, vA, g.ncols)
+        d[f"p{k}/sart_20"] = ref.sirt(0, "sart", b, 20, lb=0.0, ub=1.0, relax=1.0)
+        print("C5 group problem", k, g, flush=True)
+    g = cf.C3
+    IA, JA, vA = build(ref, 0, g.beam, g.n, g.p, g.nth, g.r, g.dw, g.sd, g.theta())
+    b = cf.add_noise(ref.matvec(0, cf.threephases(g.n, seed=5)), 0.01, 5)
+    d["kacz/b_sha"] = np.array(sha(b))
+    nord = int(np.count_nonzero(np.diff(IA) > 0))
+    for name, meth, rp in (("randkaczmarz_rowprob", "randkaczmarz", True), ("symkaczmarz", "symkaczmarz", False)):
+        ref.set(0, IA, JA, vA, g.ncols)
+        ref.rng_seed(C5_KACZ_SEED)
+        d[f"kacz/{name}_20"] = ref.art(0, meth, b, [20], lb=0.0, ub=1.0, rowprob=rp)[0]
+        print("C5 Kaczmarz", name, flush=True)
+    cnt = 2 * nord + 20 * 3 * nord
+    ref.rng_seed(C5_KACZ_SEED)
+    d["kacz/seed"] = np.array(C5_KACZ_SEED)
+    d["kacz/draws"] = np.array(cnt)
+    d["kacz/draws_sha"] = np.array(sha(ref.rng_draw(cnt)))
+    np.savez_compressed(os.path.join(HERE, "c5.npz"), **d)
+
+
 def main():
     if not Reference.available():
         sys.exit("oracle/_ref/libfair_ref.so is not built (make -C oracle ref)")
     ref = Reference()
     if "--c2" in sys.argv:
         write_c2(ref)
+        return
+    if "--c3" in sys.argv:
+        write_c3(ref)
+        return
+    if "--c5" in sys.argv:
+        write_c5(ref)
         return
     if "--twin" in sys.argv:
         write_twin(ref)
