@@ -77,6 +77,7 @@ SIGNATURES = {
     "fair_spmv_rows_dev": [_P, _P, _P],
     "fair_sirt": [_P, _I32, _P, _I32, _P, C.POINTER(Opts), _P],
     "fair_sirt_dev": [_P, _I32, _P, _I32, _P, C.POINTER(Opts)],
+    "fair_sirt_dev_noalloc": [_P, _I32, _P, _I32, _P, C.POINTER(Opts)],
     "fair_sirt_prepare": [_P, _I32],
     "fair_sirt_f32": [_P, _I32, _P, _I32, _P, C.POINTER(Opts), _P],
     "fair_sirt_f32_dev": [_P, _I32, _P, _I32, _P, C.POINTER(Opts)],

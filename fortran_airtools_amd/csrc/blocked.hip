@@ -228,6 +228,12 @@ void build_blocked_tiles(Csr &T, int32_t img_n, int32_t p, int32_t nth, int32_t 
 // can be switched off with FAIR_SPMV_STAGED=0; FAIR_DA / FAIR_DJ / FAIR_LDS_A and FAIR_TX /
 // FAIR_TY / FAIR_LDS_T override the default block shapes (fair_autotune_spmv picks them by
 // timing).
+// Whether ensure_blocked would still build a staged layout.
+bool blocked_pending(const fair_csr *h) {
+    if (h->g_n <= 0 || h->transposed || h->blk_tuned || !env_int("FAIR_SPMV_STAGED", 1)) return false;
+    return !h->A.blk.ok || !h->have_T || !h->T.blk.ok;
+}
+
 void ensure_blocked(fair_csr *h) {
     if (h->g_n <= 0 || h->transposed || h->blk_tuned || !env_int("FAIR_SPMV_STAGED", 1)) return;
     const bool built = !h->A.blk.ok || !h->have_T || !h->T.blk.ok;

@@ -207,6 +207,8 @@ void build_blocked_rays(Csr &A, int32_t img_n, int32_t p, int32_t DA, int32_t DJ
 void build_blocked_tiles(Csr &T, int32_t img_n, int32_t p, int32_t nth, int32_t TX, int32_t TY,
                          int32_t lds_elems);
 void ensure_blocked(fair_csr *h);
+bool blocked_pending(const fair_csr *h);   // ensure_blocked would still build a layout
+double *sirt_scratch_addr();               // fair_sirt_dev_noalloc's array on the current device
 // SIRT weights of sirt (AIRMs.f90:1070-1083) cached in the handle (M[method], D for SART) and the
 // device CSR of A^T; no LDS-staged layout, no workspaces.
 void sirt_weights(fair_csr *h, int method);

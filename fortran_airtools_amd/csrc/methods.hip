@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <mutex>
+
 #include "fair_internal.hpp"
 #include "spmv.hpp"
 
@@ -836,16 +838,18 @@ void sirt_setup(fair_csr *h, int method) {
     if (h->ws_x.n < (size_t)(n > 0 ? n : 1)) h->ws_x.alloc(n > 0 ? n : 1);
 }
 
-// maxits SIRT iterations on d_x (in: start, out: result), two launches per iteration.
-void sirt_iterate(fair_csr *h, int method, const double *d_b, int maxits, double *d_x, const fair_opts *o) {
+// maxits SIRT iterations on d_x (in: start, out: result), two launches per iteration; r (m) and
+// xw (n) are the workspaces: the residual and the second iterate.  Every entry of r that A^T
+// reads (the non-empty rows) and every entry of xw is written before it is read.
+void sirt_iterate(fair_csr *h, int method, const double *d_b, int maxits, double *d_x, const fair_opts *o,
+                  double *r, double *xw) {
     const Csr &A = h->A, &T = h->T;
     const int32_t n = A.n;
     const double *M = h->M[method].p;
     const double *D = method == FAIR_SART ? h->D.p : nullptr;
     const double relax = o ? o->relaxpar : 1.0;
     const Box box = make_box(o);
-    double *r = h->ws_r.p;
-    double *cur = d_x, *nxt = h->ws_x.p;
+    double *cur = d_x, *nxt = xw;
     for (int it = 0; it < maxits; ++it) {
         spmv_launch(A, cur, EpiResidual{d_b, M, r});
         EpiSirtUpdate up{cur, D, nxt, relax, box};
@@ -863,8 +867,41 @@ void sirt_iterate(fair_csr *h, int method, const double *d_b, int maxits, double
 
 void sirt_run(fair_csr *h, int method, const double *d_b, int maxits, double *d_x, const fair_opts *o) {
     sirt_setup(h, method);
-    sirt_iterate(h, method, d_b, maxits, d_x, o);
+    sirt_iterate(h, method, d_b, maxits, d_x, o, h->ws_r.p, h->ws_x.p);
 }
+
+// ---- SIRT without device allocation --------------------------------------------------------
+// Workspaces of fair_sirt_dev_noalloc: the residual (m doubles) and, from the next 256-byte
+// boundary, the second iterate (n doubles).  One array per GPU, held until the process ends;
+// every call on a GPU shares it, so calls that overlap on one GPU race.  Host code never uses
+// the name: its device address is looked up per device and kept in g_scratch_addr.
+constexpr size_t SCRATCH_DOUBLES = size_t(4) << 20;   // 32 MiB
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+double *g_scratch_addr[SCRATCH_MAX_DEVICES] = {};
+std::mutex g_scratch_mutex;
+
+size_t scratch_r_len(int32_t m) { return ((size_t)std::max(m, 1) + 31) / 32 * 32; }
+
+}  // namespace
+
+// The scratch array's address on the current device (looked up once per device).
+double *sirt_scratch_addr() {
+    int dev = 0;
+    FAIR_HIP(hipGetDevice(&dev));
+    FAIR_CHECK(dev >= 0 && dev < SCRATCH_MAX_DEVICES, FAIR_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        FAIR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        FAIR_CHECK(p != nullptr, FAIR_ERR_HIP, "no device address for the SIRT scratch array");
+        g_scratch_addr[dev] = static_cast<double *>(p);
+    }
+    return g_scratch_addr[dev];
+}
+
+namespace {
 
 }  // namespace
 
@@ -881,6 +918,27 @@ extern "C" int fair_sirt_dev(fair_csr *h, int32_t method, const double *d_b, int
         check_handle(h);
         FAIR_CHECK(maxits >= 0, FAIR_ERR_ARG, "negative iteration count");
         sirt_run(h, method, d_b, maxits, d_x, opts);
+    });
+}
+
+// fair_sirt_dev on the static scratch array: no allocation, so the matrix must already hold what
+// sirt sets up (weights, CSR of A^T, staged layouts: fair_sirt_prepare), and m + n (the residual
+// rounded up to 32 entries) must fit the array.
+extern "C" int fair_sirt_dev_noalloc(fair_csr *h, int32_t method, const double *d_b, int32_t maxits, double *d_x,
+                                     const fair_opts *opts) {
+    return guard([&] {
+        check_handle(h);
+        FAIR_CHECK(maxits >= 0, FAIR_ERR_ARG, "negative iteration count");
+        FAIR_CHECK(method == FAIR_CIMMINO || method == FAIR_SART, FAIR_ERR_METHOD, "Not a recognised SIRT method.");
+        const size_t rlen = scratch_r_len(h->A.m);
+        FAIR_CHECK(rlen + (size_t)std::max(h->A.n, 1) <= SCRATCH_DOUBLES, FAIR_ERR_ARG,
+                   "m + n exceeds the static SIRT scratch (" + std::to_string(SCRATCH_DOUBLES) +
+                       " doubles); use fair_sirt_dev");
+        FAIR_CHECK(h->w_ok[method] && h->have_T && h->A.n_rows_nz >= 0 && h->T.n_rows_nz >= 0 && !blocked_pending(h),
+                   FAIR_ERR_ARG, "matrix not prepared for this method: call fair_sirt_prepare first");
+        double *s = sirt_scratch_addr();
+        if (maxits > 0) FAIR_CHECK(d_b && d_x, FAIR_ERR_ARG, "null argument");
+        sirt_iterate(h, method, d_b, maxits, d_x, opts, s, s + rlen);
     });
 }
 

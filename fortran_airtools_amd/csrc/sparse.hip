@@ -409,6 +409,7 @@ extern "C" int fair_init(int device) {
         hipDeviceProp_t prop;
         FAIR_HIP(hipGetDeviceProperties(&prop, device));
         g_lds_per_block = (int)prop.sharedMemPerBlock;
+        sirt_scratch_addr();   // the no-allocation SIRT's array on this device
     });
 }
 

@@ -138,6 +138,15 @@ int fair_sirt(fair_csr *A, int32_t method, const double *b, int32_t maxits, cons
  * library stream once the weights exist. */
 int fair_sirt_dev(fair_csr *A, int32_t method, const double *d_b, int32_t maxits, double *d_x,
                   const fair_opts *opts);
+/* fair_sirt_dev without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The residual and the second iterate live in a library-static device
+ * array of 4 Mi doubles (32 MiB) per GPU, held until the process ends: m + n (the residual
+ * rounded up to 32 entries) must fit it, else FAIR_ERR_ARG.  A must already hold what sirt sets
+ * up for this method (fair_sirt_prepare), else FAIR_ERR_ARG.  Calls on one GPU share the array:
+ * calls that overlap on one GPU race.  Same kernels and results as fair_sirt_dev. */
+int fair_sirt_dev_noalloc(fair_csr *A, int32_t method, const double *d_b, int32_t maxits, double *d_x,
+                          const fair_opts *opts);
 /* Compute and cache the SIRT weights of sirt (AIRMs.f90:1070-1083) for A (SART also re-sorts
  * A's rows as the reference's double transpose does), so later fair_sirt_dev calls only
  * iterate. */
