@@ -22,6 +22,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -47,17 +49,23 @@ def measured_traffic() -> int | None:
     return None
 
 
-def cpu_baseline(iters: int, method: str = "sart", geom=None) -> dict | None:
-    """The reference's own method (oracle/_ref, OpenMP, up to 16 host threads) on a fan-beam
-    problem (default N=512, 720 x 725 rays; `geom` a configs.Geometry), timed over a bounded
-    number of iterations."""
+def run_cpu_baseline(geom=None, legs=("sart@16",), time_build=False, iters: int = 3) -> dict | None:
+    """The reference's own methods (oracle/_ref, OpenMP) on a fan-beam problem (default N=512,
+    720 x 725 rays; `geom` a configs.Geometry), one leg per "method@threads" (threads capped at
+    the 16 host cores a one-GPU box gives a job, and at the affinity count), each timed over a
+    bounded number of iterations; with time_build also the reference's serial fanlineartomo."""
     try:
-        cores = len(os.sched_getaffinity(0))
+        avail = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--iters", str(iters), "--threads", str(cores),
-           "--method", method]
+        avail = os.cpu_count() or 1
+    cap = max(1, min(avail, 16))
+    fixed = []
+    for leg in legs:
+        meth, _, thr = leg.partition("@")
+        fixed.append(f"{meth}@{min(int(thr or 1), cap)}")
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--iters", str(iters), "--legs", ",".join(fixed)]
+    if time_build:
+        cmd.append("--time-build")
     if geom is not None:
         cmd += ["--n", str(geom.n), "--p", str(geom.p), "--nth", str(geom.nth), "--step", str(geom.deg_step),
                 "--r", str(geom.r), "--dw", str(geom.dw), "--sd", str(geom.sd)]
@@ -70,29 +78,163 @@ def cpu_baseline(iters: int, method: str = "sart", geom=None) -> dict | None:
 
     env = dict(os.environ, OMP_STACKSIZE="256M")
     try:
-        r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=600,
+        # its progress lines pass through on stderr; the result is the last stdout line
+        r = subprocess.run(cmd, cwd=ROOT, stdout=subprocess.PIPE, text=True, timeout=600,
                            preexec_fn=big_stack, env=env)
     except subprocess.TimeoutExpired:
         log("cpu baseline timed out")
         return None
     if r.returncode != 0:
-        log(f"cpu baseline failed (status {r.returncode}):", r.stderr[-2000:])
+        log(f"cpu baseline failed (status {r.returncode})")
         return None
-    d = json.loads(r.stdout.strip().splitlines()[-1])
-    if d is None:                       # no compiled reference for this method
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def leg(d: dict | None, i: int) -> dict | None:
+    """The i-th asked leg of a run_cpu_baseline result (None when absent)."""
+    if not d:
         return None
-    cpu_model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {"value": round(d["iter_per_s"], 4), "unit": "iter/s", "cores": d["cores"],
-            "kind": d["kind"], "sample": d["sample"] + f"; {cpu_model}",
-            "gbs": None if d["gbs"] is None else round(d["gbs"], 2)}
+    vals = list(d["legs"].values())
+    return vals[i] if i < len(vals) else None
+
+
+def cpu_baseline(iters: int, method: str = "sart", geom=None) -> dict | None:
+    """One leg (the method on up to 16 host threads) in the bench line's cpu_baseline form."""
+    d = run_cpu_baseline(geom, (f"{method}@16",), iters=iters)
+    lg = leg(d, 0)
+    if lg is None:
+        return None
+    return {"value": round(lg["iter_per_s"], 4), "unit": "iter/s", "cores": lg["cores"], "kind": d["kind"],
+            "sample": lg["sample"] + f"; {d['cpu_model']}",
+            "gbs": None if lg["gbs"] is None else round(lg["gbs"], 2),
+            "nproc": d["nproc"], "affinity": d["affinity"]}
+
+
+def eq_precision(torch, _lib, L, A, b, oc, stream, iters: int = 200) -> dict:
+    """FP32 (values, iterate, accumulation; FP64 weights) against the plain FP64 SART, both from
+    x0 = 0 for `iters` iterations on HBM-resident inputs: device time of each (HIP events on the
+    library's stream) and the relative error of the FP32 iterate to the FP64 one."""
+    dev = torch.device("cuda:0")
+    b64 = torch.from_numpy(b).to(dev)
+    b32 = b64.float()
+    x64 = torch.zeros(A.n, dtype=torch.float64, device=dev)
+    x32 = torch.zeros(A.n, dtype=torch.float32, device=dev)
+    _lib.check(L.fair_sirt_f32_dev(A.handle, 1, C.c_void_p(b32.data_ptr()), 1, C.c_void_p(x32.data_ptr()),
+                                   C.byref(oc)))          # FP32 values and workspaces, untimed
+    x32.zero_()
+    torch.cuda.synchronize()
+    e0, e1 = _events(torch, stream)
+    e0.record(stream)
+    _lib.check(L.fair_sirt_dev(A.handle, 1, C.c_void_p(b64.data_ptr()), iters, C.c_void_p(x64.data_ptr()),
+                               C.byref(oc)))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t64 = e0.elapsed_time(e1) / 1e3
+    e0.record(stream)
+    _lib.check(L.fair_sirt_f32_dev(A.handle, 1, C.c_void_p(b32.data_ptr()), iters, C.c_void_p(x32.data_ptr()),
+                                   C.byref(oc)))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t32 = e0.elapsed_time(e1) / 1e3
+    err = float(torch.linalg.norm(x32.double() - x64) / torch.linalg.norm(x64))
+    log(f"C2 SART {iters} it: FP64 {t64 * 1e3:.2f} ms, FP32 {t32 * 1e3:.2f} ms, rel. error {err:.2e}")
+    return {"workload": f"N=512 fan 720 x 725, SART {iters} iterations, box [0,1]", "iterations": iters,
+            "fp64_s": round(t64, 6), "fp32_s": round(t32, 6), "fp32_relerr_vs_fp64": err,
+            "within_tolerance": err <= 1e-5, "speedup": round(t64 / t32, 3)}
+
+
+def eq_batch(torch, _lib, L, airms, cf, sg, stream, iters: int = 300, k: int = 256) -> dict:
+    """256 slices of 256^2 sharing one fan-beam matrix (360 x 363 rays): one batched FP32 SART run
+    (SpMM, tuned row orders, HIP graph) against the plain FP64 method slice by slice; per-slice
+    relative error of the batched iterate to the FP64 one."""
+    g = cf.C3
+    A = sg.fanlineartomo(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta()))
+    gens = [cf.threephases, cf.bubbles, cf.smooth]
+    dev = torch.device("cuda:0")
+    B = np.stack([cf.add_noise(A.matvecmul(gens[j % 3](g.n, seed=3 + j)), 0.01, 3 + j) for j in range(k)], 1)
+    oc = airms.Opts().set_lbound(0.0).set_ubound(1.0)._c()
+    dB = torch.from_numpy(B.astype(np.float32)).to(dev)
+    dX = torch.zeros((A.n, k), dtype=torch.float32, device=dev)
+    _lib.check(L.fair_sirt_prepare(A.handle, 1))
+    order = (C.c_int32 * 2)()
+    _lib.check(L.fair_autotune_batch(A.handle, k, order))
+    torch.cuda.synchronize()
+    for _ in range(2):      # capture (first call) and one replay, untimed
+        dX.zero_()
+        torch.cuda.synchronize()
+        _lib.check(L.fair_sirt_batch_dev(A.handle, 1, C.c_void_p(dB.data_ptr()), k, iters,
+                                         C.c_void_p(dX.data_ptr()), C.byref(oc), 1))
+    dX.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.fair_sirt_batch_dev(A.handle, 1, C.c_void_p(dB.data_ptr()), k, iters,
+                                     C.c_void_p(dX.data_ptr()), C.byref(oc), 1))
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    dB64 = torch.from_numpy(B).to(dev)
+    Xref = torch.zeros((A.n, k), dtype=torch.float64, device=dev)
+    col = torch.zeros(A.n, dtype=torch.float64, device=dev)
+    bcol = torch.zeros(A.m, dtype=torch.float64, device=dev)
+    t64 = 0.0
+    e0, e1 = _events(torch, stream)
+    for j in range(k):
+        bcol.copy_(dB64[:, j])
+        col.zero_()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        _lib.check(L.fair_sirt_dev(A.handle, 1, C.c_void_p(bcol.data_ptr()), iters, C.c_void_p(col.data_ptr()),
+                                   C.byref(oc)))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t64 += e0.elapsed_time(e1) / 1e3
+        Xref[:, j] = col
+    err = np.empty(k)
+    _lib.check(L.fair_batch_relerr_dev(C.c_void_p(dX.data_ptr()), C.c_void_p(Xref.data_ptr()), A.n, k,
+                                       err.ctypes.data_as(C.c_void_p)))
+    nnz = A.nnz
+    log(f"C3 batched FP32 SART {k} slices x {iters} it (graph): {t_batch:.3f} s; FP64 slice by slice "
+        f"{t64:.3f} s; rel. error max {err.max():.2e}")
+    return {"workload": f"{k} slices of 256x256, fan 360 x 363, SART as SpMM, {iters} iterations, HIP graph",
+            "slices": k, "iterations": iters, "nnz": nnz, "batched_fp32_s": round(t_batch, 4),
+            "fp64_slice_by_slice_s": round(t64, 4), "speedup": round(t64 / t_batch, 3),
+            "ms_per_batched_iteration": round(t_batch / iters * 1e3, 4),
+            "relerr_vs_fp64": {"max": float(err.max()), "median": float(np.median(err))},
+            "within_tolerance": bool((err <= 1e-5).all()),
+            "achieved_tflops_fp32": round(4 * nnz * k * iters / t_batch / 1e12, 3),
+            "row_orders": [order[0], order[1]]}
+
+
+def eq_matfree_n512(torch, airms, cf, sg, A, iters: int = 100) -> dict:
+    """Gradient descent on the N=512 problem (720 x 725 rays, grains phantom, box [0,1]): the
+    plain FP64 CSR method against the matrix-free FP32 projector from the same sigma, the
+    relative error of the matrix-free iterate to the FP64 one."""
+    from fortran_airtools_amd.matrix_free import FanProjector
+    g = cf.C2
+    dev = torch.device("cuda:0")
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    b = A.matvecmul(cf.grains(g.n, 200, 2))
+    sigma = airms.normest(A, v0=np.random.default_rng(2).random(A.n))
+    airms.gradient_descent(A, b, 1, tolerance=0.0, options=o, sigma=sigma)      # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x64 = airms.gradient_descent(A, b, iters, tolerance=0.0, options=o, sigma=sigma)
+    t64 = time.perf_counter() - t0
+    P = FanProjector(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta()))
+    db = torch.from_numpy(b.astype(np.float32)).to(dev)
+    x32 = torch.zeros(A.n, dtype=torch.float32, device=dev)
+    P.gradient_descent_dev(db.data_ptr(), x32.data_ptr(), max_iter=1, tolerance=0.0, sigma=sigma, options=o)
+    x32.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P.gradient_descent_dev(db.data_ptr(), x32.data_ptr(), max_iter=iters, tolerance=0.0, sigma=sigma, options=o)
+    torch.cuda.synchronize()
+    t32 = time.perf_counter() - t0
+    err = float(np.linalg.norm(x32.cpu().numpy().astype(np.float64) - x64) / np.linalg.norm(x64))
+    P.close()
+    log(f"N=512 GD {iters} it: FP64 CSR {t64:.3f} s, matrix-free FP32 {t32:.3f} s, rel. error {err:.2e}")
+    return {"workload": f"N=512 fan 720 x 725, gradient descent {iters} iterations, box [0,1]",
+            "iterations": iters, "fp64_csr_s": round(t64, 4), "matrix_free_fp32_s": round(t32, 4),
+            "relerr_vs_fp64": err, "within_tolerance": err <= 1e-5}
 
 
 def main() -> int:
@@ -101,6 +243,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-equal-error", action="store_true",
+                    help="skip the time-to-equal-error runs (FP32 at N=512, the 256-slice batch, matrix-free GD)")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--mode", default="sart",
                     choices=["sart", "precision", "batch", "matfree", "kaczmarz", "uneven", "phantoms"],
@@ -219,9 +363,47 @@ def main() -> int:
         torch.cuda.synchronize()
         del big
 
+    # time to equal error (<= 1e-5 relative to the plain FP64 iterate): FP32 at N=512, the batched
+    # FP32 run of 256 slices, matrix-free FP32 gradient descent at N=512
+    eq = None
+    if not a.no_equal_error:
+        eq = {"tolerance": 1e-5, "reference": "plain FP64 device method at the same iteration count"}
+        eq["c2_fp32_vs_fp64"] = eq_precision(torch, _lib, L, A, b, oc, stream, 200)
+        eq["n512_matrix_free_vs_fp64_csr"] = eq_matfree_n512(torch, airms, cf, sg, A, 100)
+        eq["c3_batched_fp32"] = eq_batch(torch, _lib, L, airms, cf, sg, stream, 300, 256)
+
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(a.cpu_iters)
+        # the reference on the host: SART on up to 16 threads and on one, gradient descent, and
+        # its serial matrix build, at N=512; SART at N=256 for the batched problem
+        d2 = run_cpu_baseline(None, ("sart@16", "sart@1", "gd@16"), time_build=True, iters=a.cpu_iters)
+        s16, s1, gd16 = leg(d2, 0), leg(d2, 1), leg(d2, 2)
+        if s16 is not None:
+            cpu = {"value": round(s16["iter_per_s"], 4), "unit": "iter/s", "cores": s16["cores"],
+                   "kind": d2["kind"], "sample": s16["sample"] + f"; {d2['cpu_model']}",
+                   "gbs": None if s16["gbs"] is None else round(s16["gbs"], 2),
+                   "nproc": d2["nproc"], "affinity": d2["affinity"],
+                   "value_1thread": None if s1 is None else round(s1["iter_per_s"], 4),
+                   "reference_build_s": (None if d2.get("reference_build_s") is None
+                                         else round(d2["reference_build_s"], 3))}
+        if eq is not None:
+            e = eq["c2_fp32_vs_fp64"]
+            if s16 is not None:
+                e["cpu_reference_s"] = round(e["iterations"] / s16["iter_per_s"], 3)
+                e["cpu_cores"] = s16["cores"]
+            if s1 is not None:
+                e["cpu_reference_1thread_s"] = round(e["iterations"] / s1["iter_per_s"], 3)
+            e = eq["n512_matrix_free_vs_fp64_csr"]
+            if gd16 is not None:
+                e["cpu_reference_s"] = round(e["iterations"] / gd16["iter_per_s"], 3)
+                e["cpu_cores"] = gd16["cores"]
+            d3 = run_cpu_baseline(cf.C3, ("sart@16",), iters=a.cpu_iters)
+            s3 = leg(d3, 0)
+            e = eq["c3_batched_fp32"]
+            if s3 is not None:
+                e["cpu_reference_s"] = round(e["slices"] * e["iterations"] / s3["iter_per_s"], 2)
+                e["cpu_cores"] = s3["cores"]
+                e["cpu_sample"] = s3["sample"]
 
     out = {
         "metric": METRIC, "value": round(it_s, 3), "unit": "iter/s", "n_gpus": 1,
@@ -239,6 +421,7 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "avg_launch_us": round(t_spmv * 1e6, 3), "algorithmic_bytes": bytes_spmv},
         "cpu_baseline": cpu,
+        "equal_error": eq,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -250,131 +433,37 @@ def _events(torch, stream):
 
 def bench_precision(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     """FP32 (values, iterate, accumulation; FP64 weights) against the plain FP64 method on the
-    N=512 problem: relative error of the FP32 iterate to the FP64 iterate after k iterations and
-    the time each takes, so time to equal error (<= 1e-5) can be read off."""
+    N=512 problem at several iteration counts, so time to equal error (<= 1e-5) can be read off."""
     g = cf.C2
     A = sg.fanlineartomo(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta()))
     b = cf.add_noise(A.matvecmul(cf.grains(g.n, 200, 2)), 0.005, 2)
     oc = airms.Opts().set_lbound(0.0).set_ubound(1.0)._c()
-    dev = torch.device("cuda:0")
-    b64 = torch.from_numpy(b).to(dev)
-    b32 = b64.float()
     _lib.check(L.fair_sirt_prepare(A.handle, 1))
-    rows = []
-    for k in (25, 50, 100, 200):
-        x64 = torch.zeros(A.n, dtype=torch.float64, device=dev)
-        x32 = torch.zeros(A.n, dtype=torch.float32, device=dev)
-        _lib.check(L.fair_sirt_f32_dev(A.handle, 1, C.c_void_p(b32.data_ptr()), 1, C.c_void_p(x32.data_ptr()),
-                                       C.byref(oc)))
-        x32.zero_()
-        torch.cuda.synchronize()
-        e0, e1 = _events(torch, stream)
-        e0.record(stream)
-        _lib.check(L.fair_sirt_dev(A.handle, 1, C.c_void_p(b64.data_ptr()), k, C.c_void_p(x64.data_ptr()),
-                                   C.byref(oc)))
-        e1.record(stream)
-        torch.cuda.synchronize()
-        t64 = e0.elapsed_time(e1) / 1e3
-        e0.record(stream)
-        _lib.check(L.fair_sirt_f32_dev(A.handle, 1, C.c_void_p(b32.data_ptr()), k, C.c_void_p(x32.data_ptr()),
-                                       C.byref(oc)))
-        e1.record(stream)
-        torch.cuda.synchronize()
-        t32 = e0.elapsed_time(e1) / 1e3
-        err = float(torch.linalg.norm(x32.double() - x64) / torch.linalg.norm(x64))
-        rows.append({"iterations": k, "fp64_s": round(t64, 6), "fp32_s": round(t32, 6),
-                     "fp32_relerr_vs_fp64": err, "fp32_within_1e-5": err <= 1e-5})
-        log(f"k={k}: FP64 {t64 * 1e3:.2f} ms, FP32 {t32 * 1e3:.2f} ms, rel. error {err:.2e}")
-    k = rows[-1]["iterations"]
+    rows = [eq_precision(torch, _lib, L, A, b, oc, stream, k) for k in (25, 50, 100, 200)]
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_iters)
+    last = rows[-1]
     out = {"metric": METRIC, "mode": "precision", "unit": "s", "dtype": "f32 vs f64",
-           "config": {"workload": "N=512 fan beam, 720 x 725 rays, SART, box [0,1]"},
-           "rows": rows,
-           "time_to_equal_error": {"iterations": k, "fp64_s": rows[-1]["fp64_s"], "fp32_s": rows[-1]["fp32_s"],
-                                   "speedup": round(rows[-1]["fp64_s"] / rows[-1]["fp32_s"], 3),
-                                   "fp32_relerr": rows[-1]["fp32_relerr_vs_fp64"],
-                                   "reference_cpu_s": None if cpu is None else round(k / cpu["value"], 3)},
+           "config": {"workload": "N=512 fan beam, 720 x 725 rays, SART, box [0,1]"}, "rows": rows,
+           "time_to_equal_error": dict(last, reference_cpu_s=None if cpu is None
+                                       else round(last["iterations"] / cpu["value"], 3)),
            "cpu_baseline": cpu}
-    if cpu is not None:
-        log(f"reference on {cpu['cores']} host threads: {k} SART iterations in {k / cpu['value']:.2f} s "
-            f"(from its measured {cpu['value']} it/s)")
     print(json.dumps(out), flush=True)
     return 0
 
 
 def bench_batch(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     """256 slices of 256^2 sharing one fan-beam matrix (360 angles x 363 rays): one batched FP32
-    SIRT (SART weights) run of 300 iterations captured in a HIP graph, against the plain FP64
-    method run slice by slice; per-slice relative error of the batched result."""
-    g = cf.C3
-    k, iters = 256, a.batch_iters
-    A = sg.fanlineartomo(sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta()))
-    gens = [cf.threephases, cf.bubbles, cf.smooth]
-    dev = torch.device("cuda:0")
-    X = np.stack([gens[j % 3](g.n, seed=3 + j) for j in range(k)], 1)
-    B = np.stack([cf.add_noise(A.matvecmul(X[:, j]), 0.01, 3 + j) for j in range(k)], 1)
-    oc = airms.Opts().set_lbound(0.0).set_ubound(1.0)._c()
-    dB = torch.from_numpy(B.astype(np.float32)).to(dev)
-    dX = torch.zeros((A.n, k), dtype=torch.float32, device=dev)
-    _lib.check(L.fair_sirt_prepare(A.handle, 1))
-    kc = (C.c_int32 * 2)()
-    _lib.check(L.fair_autotune_batch(A.handle, k, kc))
-    log(f"autotuned right-hand sides per wavefront: A*X {kc[0]}, A^T*R {kc[1]}")
-    torch.cuda.synchronize()
-    # capture (first call) + one replay untimed, then the timed replay
-    for _ in range(2):
-        dX.zero_()
-        torch.cuda.synchronize()
-        _lib.check(L.fair_sirt_batch_dev(A.handle, 1, C.c_void_p(dB.data_ptr()), k, iters,
-                                         C.c_void_p(dX.data_ptr()), C.byref(oc), 1))
-    dX.zero_()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    _lib.check(L.fair_sirt_batch_dev(A.handle, 1, C.c_void_p(dB.data_ptr()), k, iters,
-                                     C.c_void_p(dX.data_ptr()), C.byref(oc), 1))
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
-    log(f"batched FP32 SART, {k} slices x {iters} iterations (graph): {t_batch:.3f} s")
-    # plain FP64 method slice by slice (the per-slice reference iterates)
-    dB64 = torch.from_numpy(B).to(dev)
-    Xref = torch.zeros((A.n, k), dtype=torch.float64, device=dev)
-    col = torch.zeros(A.n, dtype=torch.float64, device=dev)
-    bcol = torch.zeros(A.m, dtype=torch.float64, device=dev)
-    t64 = 0.0
-    e0, e1 = _events(torch, stream)
-    for j in range(k):
-        bcol.copy_(dB64[:, j])
-        col.zero_()
-        torch.cuda.synchronize()
-        e0.record(stream)
-        _lib.check(L.fair_sirt_dev(A.handle, 1, C.c_void_p(bcol.data_ptr()), iters, C.c_void_p(col.data_ptr()),
-                                   C.byref(oc)))
-        e1.record(stream)
-        torch.cuda.synchronize()
-        t64 += e0.elapsed_time(e1) / 1e3
-        Xref[:, j] = col
-    err = np.empty(k)
-    _lib.check(L.fair_batch_relerr_dev(C.c_void_p(dX.data_ptr()), C.c_void_p(Xref.data_ptr()), A.n, k,
-                                       err.ctypes.data_as(C.c_void_p)))
-    log(f"FP64 slice by slice: {t64:.3f} s; batched rel. error max {err.max():.2e} median {np.median(err):.2e}")
-    nnz = A.nnz
-    flops = 2 * 2 * nnz * k * iters
+    SIRT (SART weights) run captured in a HIP graph against the plain FP64 method slice by slice,
+    with the reference's SART timed on the host (per slice, scaled to the batch)."""
+    r = eq_batch(torch, _lib, L, airms, cf, sg, stream, a.batch_iters, 256)
     out = {"metric": METRIC, "mode": "batch", "unit": "slice-iterations/s", "dtype": "f32 (fp64 weights)",
-           "value": round(k * iters / t_batch, 2),
-           "config": {"workload": f"256 slices of 256x256, fan beam 360 x 363 rays, SART as SpMM, {iters} iterations, "
-                                  "HIP graph", "nnz": nnz, "slices": k, "iterations": iters},
-           "batched_fp32_s": round(t_batch, 4), "fp64_slice_by_slice_s": round(t64, 4),
-           "speedup_vs_fp64": round(t64 / t_batch, 3),
-           "relerr_vs_fp64": {"max": float(err.max()), "median": float(np.median(err)),
-                              "all_within_1e-5": bool((err <= 1e-5).all())},
-           "achieved_tflops_fp32": round(flops / t_batch / 1e12, 3),
-           "autotuned_rhs_per_wavefront": [kc[0], kc[1]]}
-    cpu = None if a.no_cpu else cpu_baseline(a.cpu_iters, geom=g)
+           "value": round(r["slices"] * r["iterations"] / r["batched_fp32_s"], 2), **r}
+    cpu = None if a.no_cpu else cpu_baseline(a.cpu_iters, geom=cf.C3)
     if cpu is not None:
         out["cpu_baseline"] = cpu
-        out["reference_cpu_s"] = round(k * iters / cpu["value"], 2)
+        out["reference_cpu_s"] = round(r["slices"] * r["iterations"] / cpu["value"], 2)
         log(f"reference on {cpu['cores']} host threads: {cpu['value']} slice-iterations/s, "
-            f"{k * iters / cpu['value']:.1f} s for the whole batch")
+            f"{out['reference_cpu_s']:.1f} s for the whole batch")
     print(json.dumps(out), flush=True)
     return 0
 
@@ -440,30 +529,8 @@ def bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     P.close()
     # N=512: FP64 CSR gradient descent against matrix-free FP32 from the same sigma
     g2 = cf.C2
-    d2 = sg.Data(n=g2.n, p=g2.p, nth=g2.nth, r=g2.r, dw=g2.dw, sd=g2.sd, theta=g2.theta())
-    A = sg.fanlineartomo(d2)
-    xt2 = cf.grains(g2.n, 200, 2)
-    b2 = A.matvecmul(xt2)
-    sigma2 = airms.normest(A, v0=np.random.default_rng(2).random(A.n))
-    airms.gradient_descent(A, b2, 1, tolerance=0.0, options=o, sigma=sigma2)      # warm-up: transpose
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    x64 = airms.gradient_descent(A, b2, iters, tolerance=0.0, options=o, sigma=sigma2)
-    t64 = time.perf_counter() - t0
-    P2 = FanProjector(d2)
-    db2 = torch.from_numpy(b2.astype(np.float32)).to(dev)
-    x32 = torch.zeros(A.n, dtype=torch.float32, device=dev)
-    P2.gradient_descent_dev(db2.data_ptr(), x32.data_ptr(), max_iter=1, tolerance=0.0, sigma=sigma2, options=o)
-    x32.zero_()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    P2.gradient_descent_dev(db2.data_ptr(), x32.data_ptr(), max_iter=iters, tolerance=0.0, sigma=sigma2,
-                            options=o)
-    torch.cuda.synchronize()
-    t32 = time.perf_counter() - t0
-    xh = x32.cpu().numpy().astype(np.float64)
-    err = float(np.linalg.norm(xh - x64) / np.linalg.norm(x64))
-    log(f"N=512 GD {iters} it: FP64 CSR {t64:.3f} s, matrix-free FP32 {t32:.3f} s, rel. error {err:.3e}")
+    A = sg.fanlineartomo(sg.Data(n=g2.n, p=g2.p, nth=g2.nth, r=g2.r, dw=g2.dw, sd=g2.sd, theta=g2.theta()))
+    n512 = eq_matfree_n512(torch, airms, cf, sg, A, iters)
     cpu = None if a.no_cpu else cpu_baseline(a.cpu_iters, "gd")
     out = {"metric": METRIC, "mode": "matfree", "unit": "GD iter/s", "dtype": "f32 (fp64 ray geometry and norms)",
            "value": round(its / t_gd, 3),
@@ -476,9 +543,7 @@ def bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
                               "gd": round(2 * nseg * its / t_gd, 1)},
            "normest_s": round(t_norm, 4), "sigma": sigma, "gd_s": round(t_gd, 4),
            "relerr_to_phantom": err_truth, "dot_product_relerr": max(dots),
-           "time_to_equal_error_n512": {"iterations": iters, "fp64_csr_s": round(t64, 4),
-                                        "matrix_free_fp32_s": round(t32, 4), "relerr": err,
-                                        "within_1e-5": err <= 1e-5},
+           "time_to_equal_error_n512": n512,
            "cpu_baseline_n512_gd": cpu}
     print(json.dumps(out), flush=True)
     return 0

@@ -297,6 +297,7 @@ class Reference:
         L.ref_quicksort.argtypes = [i, _F64P, _I32P]
         L.ref_art_xref.argtypes = [i, i, i, _F64P, i, _I32P, i, i, _F64P, _F64P, i, i, d, d, d, d, i, _F64P]
         L.ref_rng_seed.argtypes = [i]
+        L.ref_set_threads.argtypes = [i]
         L.ref_rng_draw.argtypes = [i, _F64P]
         L.ref_shepplogan.argtypes = [i, _F64P]
         L.ref_phantom.argtypes = [i, i, i, i, d, i, i, _F64P]
@@ -462,6 +463,10 @@ class Reference:
 
     def rng_seed(self, seed):
         self.L.ref_rng_seed(seed)
+
+    def set_threads(self, nt):
+        """OpenMP threads of the reference's parallel regions from now on."""
+        self.L.ref_set_threads(int(nt))
 
     def rng_draw(self, cnt):
         u = np.empty(cnt)

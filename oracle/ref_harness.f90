@@ -395,6 +395,13 @@ contains
     end subroutine ref_store
 
     ! Seed the runtime generator deterministically from one integer.
+    ! OpenMP threads of the reference's parallel regions (MMRS, MMCS, compute_row_norms) from now on.
+    subroutine ref_set_threads(nt) bind(C, name='ref_set_threads')
+        use omp_lib, only: omp_set_num_threads
+        integer(c_int), value :: nt
+        call omp_set_num_threads(nt)
+    end subroutine ref_set_threads
+
     subroutine ref_rng_seed(seed) bind(C, name='ref_rng_seed')
         integer(c_int), value :: seed
         integer :: k, i
