@@ -121,9 +121,11 @@ _lock = threading.Lock()
 _ready = False
 
 
-def load(path: str = LIB_PATH) -> C.CDLL:
-    """Load the shared library (no device initialisation)."""
+def load(path: str | None = None) -> C.CDLL:
+    """Load the shared library (no device initialisation).  FAIR_HIP_LIB names another build of it
+    (the tests' sanitizer build of the host code)."""
     global _lib
+    path = path or os.environ.get("FAIR_HIP_LIB") or LIB_PATH
     with _lock:
         if _lib is None:
             if not os.path.exists(path):
@@ -132,10 +134,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
             # libamdhip64.so.7.  Loaded first, it satisfies this library's dependency and the
             # process has one runtime; loaded after /opt/rocm's, a second HSA runtime starts
             # and finds no device.  So torch, when present, is imported before the library.
-            try:
-                import torch  # noqa: F401
-            except ImportError:
-                pass
+            if not os.environ.get("FAIR_NO_TORCH"):
+                try:
+                    import torch  # noqa: F401
+                except ImportError:
+                    pass
             lib = C.CDLL(path)
             for name, args in SIGNATURES.items():
                 f = getattr(lib, name)

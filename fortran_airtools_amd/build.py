@@ -40,6 +40,47 @@ def _compile(src: str) -> str:
     return obj
 
 
+# Host code instrumented with AddressSanitizer and UndefinedBehaviorSanitizer (the device code is
+# built as usual): every sanitizer flag applies to the host pass only.
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",
+            "-Xarch_host", "-shared-libsan"]
+SAN_DIR = os.path.join(BUILD, "san")
+SAN_LIB = os.path.join(SAN_DIR, "libfair_hip_san.so")
+
+
+def _compile_san(src: str) -> str:
+    obj = os.path.join(SAN_DIR, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *COMMON, *PER_FILE.get(os.path.basename(src), []), *SAN_HOST, "-g", "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc (sanitized host) failed for {src}:\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build_hip_sanitized(verbose: bool = False) -> str:
+    """_build/san/libfair_hip_san.so: the library with sanitized host code, for the tests'
+    sanitizer run (the loading process preloads the clang sanitizer runtime).  Rebuilt only when a
+    source is newer than it."""
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".hpp")))
+    srcs.append(os.path.join(ROOT, "include", "fair_hip.h"))
+    if os.path.exists(SAN_LIB) and os.path.getmtime(SAN_LIB) >= max(os.path.getmtime(f) for f in srcs):
+        return SAN_LIB
+    os.makedirs(SAN_DIR, exist_ok=True)
+    hips = [f for f in srcs if f.endswith(".hip")]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(hips))) as ex:
+        objs = list(ex.map(_compile_san, hips))
+    tmp = SAN_LIB + ".tmp"
+    r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, SAN_LIB)
+    if verbose:
+        print("built", SAN_LIB)
+    return SAN_LIB
+
+
 def build_hip(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))

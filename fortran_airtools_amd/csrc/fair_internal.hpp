@@ -55,6 +55,11 @@ hipStream_t stream();
 void launch_check(const char *what);
 
 // ---- device memory -------------------------------------------------------------------------
+// Every device allocation of the library passes through here first.  FAIR_FAIL_ALLOC=N makes the
+// N-th allocation of the process (counted from 1) fail as an out-of-memory hipMalloc would,
+// before hipMalloc is called, so that failure paths can be tested, without a GPU too.
+void before_device_alloc(size_t bytes);
+
 template <class T>
 struct DBuf {
     T *p = nullptr;
@@ -71,8 +76,11 @@ struct DBuf {
     ~DBuf() { release(); }
     void alloc(size_t count) {
         release();
+        if (count) {
+            before_device_alloc(count * sizeof(T));
+            FAIR_HIP(hipMalloc(&p, count * sizeof(T)));
+        }
         n = count;
-        if (count) FAIR_HIP(hipMalloc(&p, count * sizeof(T)));
     }
     void release() {
         if (p) (void)hipFree(p);

@@ -10,6 +10,7 @@
 // A^T*y is a gather over a device-built CSR of A^T instead of the reference's racy scatter:
 // deterministic, and every thread owns its output.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
 
@@ -29,6 +30,15 @@ int g_lds_per_block = 65536;
 }  // namespace
 
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+void before_device_alloc(size_t bytes) {
+    static std::atomic<long long> count{0};
+    const long long k = ++count;
+    const char *v = getenv("FAIR_FAIL_ALLOC");
+    if (v && *v && atoll(v) == k)
+        throw Error{FAIR_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + " bytes): out of memory (allocation " +
+                                      std::to_string(k) + " failed by FAIR_FAIL_ALLOC)"};
+}
 hipStream_t stream() { return g_stream; }
 int lds_per_block() { return g_lds_per_block; }
 

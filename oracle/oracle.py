@@ -34,7 +34,7 @@ class Oracle:
     """C restatement of the reference path (fair_oracle.c)."""
 
     def __init__(self, path: str | None = None):
-        path = path or os.path.join(HERE, "liboracle.so")
+        path = path or os.environ.get("FAIR_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             build_oracle()
         L = C.CDLL(path)

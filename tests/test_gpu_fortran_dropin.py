@@ -12,20 +12,39 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "fortran_airtools_amd", "bin", "dropin_check")
 
 
-def test_fortran_dropin_c1(tmp_path, oracle):
+def write_input(path, gold):
     from fortran_airtools_amd import configs as cf
-    if not os.path.exists(EXE):
-        pytest.fail("fortran_airtools_amd/bin/dropin_check is not built (run __graft_entry__.build())")
-    gold = load_golden("c1.npz")
     g = cf.C1
-    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    with open(fin, "wb") as f:
+    with open(path, "wb") as f:
         np.array([g.n, g.p, g.nth], np.int32).tofile(f)
         np.array([g.r, g.dw, g.sd], np.float64).tofile(f)
         g.theta().astype(np.float64).tofile(f)
         np.array([g.m], np.int32).tofile(f)
         gold["b"].astype(np.float64).tofile(f)
         gold["probe_x"].astype(np.float64).tofile(f)
+
+
+def test_fortran_dropin_stops_on_allocation_failure(tmp_path):
+    """FAIR_FAIL_ALLOC=1 fails the program's first device allocation (the matrix build): the
+    drop-in stops with the library's message, as the reference's `stop "<message>"` does, and
+    writes no results."""
+    if not os.path.exists(EXE):
+        pytest.fail("fortran_airtools_amd/bin/dropin_check is not built (run __graft_entry__.build())")
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, load_golden("c1.npz"))
+    r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, FAIR_FAIL_ALLOC="1"))
+    out = r.stdout + r.stderr
+    assert "out of memory (allocation 1 failed by FAIR_FAIL_ALLOC)" in out, out
+    assert not fout.exists() or fout.stat().st_size == 0
+
+
+def test_fortran_dropin_c1(tmp_path, oracle):
+    if not os.path.exists(EXE):
+        pytest.fail("fortran_airtools_amd/bin/dropin_check is not built (run __graft_entry__.build())")
+    gold = load_golden("c1.npz")
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, gold)
     r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = fout.read_bytes()
