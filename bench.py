@@ -49,6 +49,22 @@ def measured_traffic() -> int | None:
     return None
 
 
+def matfree_counts() -> dict:
+    """Per-segment flop and VALU-operation counts of the matrix-free kernels from the committed
+    counter summary (profiles/matrix_free_fan_n2048_fp32_valu_counters.md, "name: value" lines)."""
+    path = os.path.join(ROOT, "profiles", "matrix_free_fan_n2048_fp32_valu_counters.md")
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                k, sep, v = line.partition(":")
+                if sep and k.strip().endswith("_per_segment"):
+                    out[k.strip()] = float(v)
+    except (OSError, ValueError):
+        pass
+    return out
+
+
 def run_cpu_baseline(geom=None, legs=("sart@16",), time_build=False, iters: int = 3) -> dict | None:
     """The reference's own methods (oracle/_ref, OpenMP) on a fan-beam problem (default N=512,
     720 x 725 rays; `geom` a configs.Geometry), one leg per "method@threads" (threads capped at
@@ -293,13 +309,21 @@ def main() -> int:
 
     g = cf.C2
     dat = sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta())
+    # the first build of a process also loads the library's code object onto the device: timed
+    # apart from a second, warm build (the reference's build is timed in a warm process too)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A = sg.fanlineartomo(dat)
+    torch.cuda.synchronize()
+    t_build_first = time.perf_counter() - t0
+    del A
     t0 = time.perf_counter()
     A = sg.fanlineartomo(dat)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     m, n, nnz = A.m, A.n, A.nnz
-    log(f"matrix {m} x {n}, nnz {nnz}, device build {t_build:.3f} s")
+    log(f"matrix {m} x {n}, nnz {nnz}, device build {t_build:.4f} s (first call of the process "
+        f"{t_build_first:.3f} s)")
 
     xt = cf.grains(g.n, 200, 2)
     b = cf.add_noise(A.matvecmul(xt), 0.005, 2)
@@ -415,7 +439,7 @@ def main() -> int:
                    "global_batch": 1, "seq_len": 0, "parallelism": "single GPU",
                    "sart_gbs_per_iteration": round(bytes_it / (t_sart / a.steps) / 1e9, 2),
                    "pcie_inclusive_iter_per_s": round(a.steps / t_host, 3),
-                   "matrix_build_s": round(t_build, 4)},
+                   "matrix_build_s": round(t_build, 4), "matrix_build_first_call_s": round(t_build_first, 4)},
         "roofline": {"kernel": "k_spmv_blocked<double, SrcPtr<double>, EpiStore<double>>",
                      "bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
@@ -538,6 +562,7 @@ def bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
                                   "gradient descent, box [0,1]", "iterations": its,
                       "segments_per_product": nseg},
            "forward_ms": round(times["forward"] * 1e3, 3), "adjoint_ms": round(times["adjoint"] * 1e3, 3),
+           "roofline": mf_roofline(nseg, times),
            "segments_per_s": {"forward": round(nseg / times["forward"], 1),
                               "adjoint": round(nseg / times["adjoint"], 1),
                               "gd": round(2 * nseg * its / t_gd, 1)},
@@ -547,6 +572,28 @@ def bench_matfree(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
            "cpu_baseline_n512_gd": cpu}
     print(json.dumps(out), flush=True)
     return 0
+
+
+FP64_VECTOR_PEAK_TFLOPS = 78.6   # also the FP32 vector rate without packed FP32 instructions
+VALU_LANE_OPS_PEAK = 256 * 64 * 2.4e9   # 256 CUs, 64 lanes a cycle, 2.4 GHz
+
+
+def mf_roofline(nseg: float, times: dict) -> dict:
+    """The matrix-free products against the vector peaks: floating-point rate (the counted flops
+    per segment times the measured segments per second) and VALU issue (lane operations)."""
+    c = matfree_counts()
+    out = {"bound": "valu", "unit": "TFLOP/s", "peak": FP64_VECTOR_PEAK_TFLOPS,
+           "valu_lane_ops_peak_per_s": VALU_LANE_OPS_PEAK}
+    for k in ("forward", "adjoint"):
+        fl, ops = c.get(f"{k}_fp_flops_per_segment"), c.get(f"{k}_valu_lane_ops_per_segment")
+        if fl is None or ops is None:
+            out[k] = None
+            continue
+        rate = nseg / times[k]
+        out[k] = {"achieved": round(fl * rate / 1e12, 3), "frac": round(fl * rate / 1e12 / FP64_VECTOR_PEAK_TFLOPS, 4),
+                  "valu_issue_frac": round(ops * rate / VALU_LANE_OPS_PEAK, 4),
+                  "flops_per_segment": fl, "valu_lane_ops_per_segment": ops}
+    return out
 
 
 def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:

@@ -225,3 +225,61 @@ def test_finite_differences_c4(c4):
         an = np.dot(g, d)
         worst = max(worst, abs(fd - an) / abs(an))
     assert worst <= 1e-4, worst
+
+
+def objective(P, x, b):
+    """1/2 ||A x - b||^2 with the projector's FP32 residual, summed in FP64."""
+    r = P.matvecmul(x).astype(np.float64) - b.astype(np.float64)
+    return 0.5 * np.dot(r, r)
+
+
+def test_gradient_descent_100_iterations_c4(c4):
+    """gradient_descent (AIRMs.f90:860-995) on the 2048^2 problem, matrix-free, box [0,1]:
+    100 accepted updates with tolerance 0 (the step rho/sigma^2, rho = 1.5, is below 2/||A||^2,
+    so the projected iteration decreases the objective), the objective after 1, 3, 10, 30, 100
+    iterations non-increasing and the iterate approaching the phantom."""
+    import torch
+    from fortran_airtools_amd import configs as cf
+    from fortran_airtools_amd.airms import Opts
+    dat, P = c4
+    xt = cf.blobs(dat.n).astype(np.float32)
+    b = cf.add_noise(P.matvecmul(xt).astype(np.float64), 0.01, 4).astype(np.float32)
+    sigma = P.normest(v0=np.random.default_rng(4).random(P.n))
+    o = Opts().set_lbound(0.0).set_ubound(1.0)
+    db = torch.from_numpy(b).cuda()
+    dx = torch.zeros(P.n, dtype=torch.float32, device="cuda")
+    f_prev, e_prev = objective(P, np.zeros(P.n, np.float32), b), 1.0
+    for k in (1, 3, 10, 30, 100):
+        dx.zero_()
+        its = P.gradient_descent_dev(db.data_ptr(), dx.data_ptr(), max_iter=k, tolerance=0.0, sigma=sigma, options=o)
+        assert its == k
+        x = dx.cpu().numpy()
+        assert x.min() >= 0.0 and x.max() <= 1.0
+        f = objective(P, x, b)
+        e = np.linalg.norm(x - xt) / np.linalg.norm(xt)
+        print(f"GD {k}: objective {f:.6e}, error to phantom {e:.4f}", flush=True)
+        assert f <= f_prev * (1 + 1e-6), (k, f, f_prev)
+        assert e < e_prev
+        f_prev, e_prev = f, e
+
+
+def test_gradient_descent_exit_iteration_equals_fp64_csr():
+    """With a tolerance, the matrix-free FP32 run stops at the same iteration as the plain FP64
+    CSR method (same sigma) on the 256^2 fan of 360 x 363 rays, and its iterate is within 1e-5."""
+    import torch
+    from fortran_airtools_amd import airms, configs as cf, system_generation as sg
+    g = cf.C3
+    dat = sg.Data(n=g.n, p=g.p, nth=g.nth, r=g.r, dw=g.dw, sd=g.sd, theta=g.theta())
+    A = sg.fanlineartomo(dat)
+    b = cf.add_noise(A.matvecmul(cf.threephases(g.n, seed=5)), 0.01, 5)
+    sigma = airms.normest(A, v0=np.random.default_rng(6).random(A.n))
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    x64, its64 = airms.gradient_descent(A, b, 2000, tolerance=5e-3, options=o, sigma=sigma, return_iters=True)
+    assert 0 < its64 < 2000
+    P = projector(dat)
+    db = torch.tensor(b, dtype=torch.float32, device="cuda")
+    dx = torch.zeros(A.n, dtype=torch.float32, device="cuda")
+    its = P.gradient_descent_dev(db.data_ptr(), dx.data_ptr(), max_iter=2000, tolerance=5e-3, sigma=sigma, options=o)
+    print(f"exit iteration: FP64 CSR {its64}, matrix-free FP32 {its}", flush=True)
+    assert its == its64
+    assert relerr(dx.cpu().numpy(), x64) <= 1e-5
