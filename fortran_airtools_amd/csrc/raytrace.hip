@@ -257,70 +257,177 @@ __device__ int exact_ray(const RayArgs &g, int64_t ray, char *base, E &e, bool *
     return s.cnt;
 }
 
-// The same algorithm with its sort in LDS, one 64-lane workgroup per ray: the lanes compute the
-// 2n+2 crossing parameters, lane 0 sorts (t, ind) with the reference's quicksort and runs the
-// point rule, recomputing each point from its index as the reference's arrays hold it
-// (system_generation.f90:162-170).  LDS: t (8 B), ind (4 B) and the range stack (8 B) per
-// crossing.  Shorter latency than global scratch by an order of magnitude.
-constexpr int EXACT_LDS_PER_CROSSING = 20;
+// The same algorithm with its sort in LDS, one 64-lane wavefront per ray.  The lanes compute the
+// 2n+2 crossing parameters; the reference's quicksort of (t, ind) (system_generation.f90:37-77:
+// Hoare partition around the middle element, both sub-ranges sorted in turn) then runs with each
+// partition spread over the wavefront.  Its inputs (the x-line and the y-line parameters, two
+// monotone runs) split very unevenly, so the partitions, not the ranges, carry the work.
+//
+// A Hoare partition of [f, l] around x swaps, while they have not crossed, the k-th "left stopper"
+// (a position the left scan stops at: !(t < x), or l) with the k-th "right stopper" from the right
+// (!(x < t), or f).  Every position a scan reaches before the two meet still holds its original
+// value, and a swapped position holds a stopper for the scan that meets it next, so: with L and R
+// the stopper positions of the original range (L increasing, R decreasing), K the first k with
+// L[k] >= R[k], the partition swaps (L[k], R[k]) for k < K and ends with i = min(L[K], R[K-1]),
+// j = max(R[K], L[K-1]) (i = L[0], j = R[0] when K = 0), exactly as the sequential loop does, NaN
+// included (a NaN compares false, so it stops both scans).  Here L and R come from wave prefix
+// sums, K from a wave vote, and the K swaps are independent.  The pending ranges are disjoint, so
+// the order they are taken in does not change the result.
+//
+// The point rule follows in sorted order (system_generation.f90:162-203): each point is
+// recomputed from its index as the reference's arrays hold it, the points inside the box are
+// compacted, and each lane takes one pair of consecutive kept points, emitting the pair's segment
+// at its prefix position among the accepted pairs.  LDS per crossing: t (8 B), the range stack
+// (8 B), ind (4 B), the stopper lists (2 x 4 B); after the sort the kept points overwrite t and
+// the stack.
+constexpr int EXACT_LDS_PER_CROSSING = 28;
+
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return __popcll(mask & ((1ull << (threadIdx.x & 63)) - 1ull));
+}
 
 template <int BEAM, bool FILL, class E>
 __device__ int exact_ray_lds(const RayArgs &g, int64_t ray, unsigned char *lds, E &e, bool *bad_col) {
     const int n = g.n, nn = 2 * n + 2;
+    const int lane = (int)threadIdx.x;
     double *t = (double *)lds - 1;                              // t[1..nn]
-    int32_t *stk = (int32_t *)(lds + 8 * (size_t)nn);          // 2*nn entries
+    int32_t *stk = (int32_t *)(lds + 8 * (size_t)nn);          // pending ranges, 2 entries each
     int32_t *ind = (int32_t *)(lds + 16 * (size_t)nn) - 1;     // ind[1..nn]
+    int32_t *Lp = (int32_t *)(lds + 20 * (size_t)nn);          // left stoppers, increasing
+    int32_t *Rp = (int32_t *)(lds + 24 * (size_t)nn);          // right stoppers, increasing
     const RayGeom r = ray_geom<BEAM>(g, ray);
     const double half = .5 * (double)n;
-    for (int k = 1 + (int)threadIdx.x; k <= n + 1; k += (int)blockDim.x) {
+    for (int k = 1 + lane; k <= n + 1; k += WAVE) {
         const double xg = gridc(k, half);
         t[k] = (xg - r.x0th) / r.a;
         t[n + 1 + k] = (xg - r.y0th) / r.b;
         ind[k] = k;
         ind[n + 1 + k] = n + 1 + k;
     }
+    if (lane == 0) {
+        stk[0] = 1;
+        stk[1] = nn;
+    }
     __syncthreads();
-    if (threadIdx.x != 0) return 0;
-    int sp = 0;
-    stk[sp++] = 1;
-    stk[sp++] = nn;
+    // pending ranges are disjoint with at least two entries each: at most nn / 2 of them
+    int sp = 1;
     while (sp > 0) {
-        int l = stk[--sp], f = stk[--sp];
-        double x = t[(f + l) / 2];
-        int i = f, j = l;
-        for (;;) {
-            while (i < l && t[i] < x) ++i;
-            while (j > f && x < t[j]) --j;
-            if (i >= j) break;
-            double tt = t[i]; t[i] = t[j]; t[j] = tt;
-            int32_t mm = ind[i]; ind[i] = ind[j]; ind[j] = mm;
-            ++i;
-            --j;
+        --sp;
+        const int f = stk[2 * sp], l = stk[2 * sp + 1];
+        const double x = t[(f + l) / 2];
+        int nL = 0, nR = 0;
+        for (int p0 = f; p0 <= l; p0 += WAVE) {
+            const int p = p0 + lane;
+            const bool in = p <= l;
+            const double v = in ? t[p] : 0.0;
+            const bool ls = in && (!(v < x) || p == l);
+            const bool rs = in && (!(x < v) || p == f);
+            const uint64_t bl = __ballot(ls), br = __ballot(rs);
+            if (ls) Lp[nL + lanes_below(bl)] = p;
+            if (rs) Rp[nR + lanes_below(br)] = p;
+            nL += __popcll(bl);
+            nR += __popcll(br);
         }
-        if (j + 1 < l) { stk[sp++] = j + 1; stk[sp++] = l; }
-        if (f < i - 1) { stk[sp++] = f; stk[sp++] = i - 1; }
-    }
-    PointSink s;
-    s.lo = -half;
-    s.hi = half;
-    s.nd = (double)n;
-    s.n = n;
-    for (int k = 1; k <= nn; ++k) {
-        const int q = ind[k];
-        double px, py;
-        if (q <= n + 1) {                       // x-line crossing (:162-164)
-            const double xg = gridc(q, half);
-            px = xg;
-            py = r.b * ((xg - r.x0th) / r.a) + r.y0th;
-        } else {                                // y-line crossing (:167-169)
-            const double yg = gridc(q - (n + 1), half);
-            px = r.a * ((yg - r.y0th) / r.b) + r.x0th;
-            py = yg;
+        __syncthreads();
+        // L holds l and R holds f, so some k < min(nL, nR) has L[k] >= R[k] (R[k] = Rp[nR-1-k])
+        const int kmax = min(nL, nR);
+        int K = kmax - 1;   // (the bound; the vote finds it at the latest there)
+        for (int k0 = 0; k0 < kmax; k0 += WAVE) {
+            const int k = k0 + lane;
+            const bool c = k < kmax && Lp[k] >= Rp[nR - 1 - k];
+            const uint64_t b = __ballot(c);
+            if (b) {
+                K = k0 + __ffsll((long long)b) - 1;
+                break;
+            }
         }
-        s.template push<FILL, E>(px, py, r.excl, e);
+        for (int k = lane; k < K; k += WAVE) {
+            const int a = Lp[k], c = Rp[nR - 1 - k];
+            const double tt = t[a];
+            t[a] = t[c];
+            t[c] = tt;
+            const int32_t mm = ind[a];
+            ind[a] = ind[c];
+            ind[c] = mm;
+        }
+        const int i = K > 0 ? min(Lp[K], Rp[nR - K]) : Lp[0];
+        const int j = K > 0 ? max(Rp[nR - 1 - K], Lp[K - 1]) : Rp[nR - 1];
+        __syncthreads();   // swaps and stopper reads done before the next partition
+        if (j + 1 < l) {
+            if (lane == 0) {
+                stk[2 * sp] = j + 1;
+                stk[2 * sp + 1] = l;
+            }
+            ++sp;
+        }
+        if (f < i - 1) {
+            if (lane == 0) {
+                stk[2 * sp] = f;
+                stk[2 * sp + 1] = i - 1;
+            }
+            ++sp;
+        }
+        __syncthreads();
     }
-    if (s.bad_col) *bad_col = true;
-    return s.cnt;
+    // the points in sorted order; those inside the box (:187-193), compacted, over t and the stack
+    double *kx = (double *)lds, *ky = (double *)(lds + 8 * (size_t)nn);
+    const double lo = -half, hi = half;
+    int nk = 0;
+    for (int k0 = 1; k0 <= nn; k0 += WAVE) {
+        const int k = k0 + lane;
+        double px = 0.0, py = 0.0;
+        bool keep = false;
+        if (k <= nn) {
+            const int q = ind[k];
+            if (q <= n + 1) {                       // x-line crossing (:162-164)
+                const double xg = gridc(q, half);
+                px = xg;
+                py = r.b * ((xg - r.x0th) / r.a) + r.y0th;
+            } else {                                // y-line crossing (:167-169)
+                const double yg = gridc(q - (n + 1), half);
+                px = r.a * ((yg - r.y0th) / r.b) + r.x0th;
+                py = yg;
+            }
+            keep = px >= lo && px <= hi && py >= lo && py <= hi;
+        }
+        const uint64_t b = __ballot(keep);
+        __syncthreads();   // ind of this chunk read before kx / ky (over t, the stack) are written
+        if (keep) {
+            const int o = nk + lanes_below(b);
+            kx[o] = px;
+            ky[o] = py;
+        }
+        nk += __popcll(b);
+    }
+    __syncthreads();
+    // consecutive kept points (previous, current): the segment test and the midpoint column rule
+    // (:196-203, :279-297)
+    const double nd = (double)n;
+    int cnt = 0;
+    bool bad = false;
+    for (int q0 = 1; q0 < nk; q0 += WAVE) {
+        const int q = q0 + lane;
+        bool val = false;
+        double tv = 0.0;
+        int col = 0;
+        if (q < nk) {
+            const double ax = kx[q - 1], ay = ky[q - 1], bx = kx[q], by = ky[q];
+            const double dx = ax - bx, dy = ay - by;
+            tv = sqrt(dx * dx + dy * dy);
+            val = tv > 1e-10 && !r.excl;
+            if (FILL && val) {
+                const double xm = .5 * ((ax + bx) + nd);
+                const double ym = .5 * ((ay + by) + nd);
+                col = (int)floor(xm) * n + (n - (int)floor(ym));
+                if (col == 0) bad = true;
+            }
+        }
+        const uint64_t b = __ballot(val);
+        if (FILL && val) e.emit(cnt + lanes_below(b), col, tv);
+        cnt += __popcll(b);
+    }
+    if (__ballot(bad)) *bad_col = true;
+    return cnt;
 }
 
 template <int BEAM, bool FILL>
