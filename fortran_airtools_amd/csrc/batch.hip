@@ -260,11 +260,17 @@ const float *zero_row() {
 
 // Row orders of the SpMM.  Code 0 keeps the compacted row list's order; codes 1.. group rays
 // (rows of a ray-traced A: angle a = r / p, detector d = r % p) into BA x BD sinogram tiles or
-// pixels (rows of A^T: image column x = j / N, row y = j % N) into TX x TY image tiles, tiles in
-// row-major order, rows inside a tile likewise.
-constexpr int SPMM_ORDERS = 8;
-constexpr int RAY_TILES[SPMM_ORDERS][2] = {{0, 0}, {16, 16}, {32, 8}, {64, 4}, {64, 8}, {128, 4}, {32, 16}, {128, 2}};
-constexpr int PIX_TILES[SPMM_ORDERS][2] = {{0, 0}, {8, 8}, {16, 16}, {32, 32}, {16, 32}, {32, 16}, {64, 64}, {16, 8}};
+// pixels (rows of A^T: image column x = j / N, row y = j % N) into TX x TY image tiles.  Tiles
+// run angle-block (column-block) major, or with the third entry 1 detector-band (row-band) major
+// (a band of detectors through all angles); rows inside a tile run row-major.
+constexpr int SPMM_ORDERS = 12;
+constexpr int ALL = 1 << 20;   // one block spanning every angle
+constexpr int RAY_TILES[SPMM_ORDERS][3] = {{0, 0, 0},   {16, 16, 0}, {32, 8, 0},   {32, 16, 0},
+                                           {64, 8, 0},  {32, 32, 0}, {64, 16, 0},  {16, 32, 0},
+                                           {32, 16, 1}, {64, 8, 1},  {ALL, 16, 1}, {ALL, 8, 1}};
+constexpr int PIX_TILES[SPMM_ORDERS][3] = {{0, 0, 0},   {8, 8, 0},   {16, 16, 0}, {32, 32, 0},
+                                           {16, 32, 0}, {32, 16, 0}, {16, 8, 0},  {8, 16, 0},
+                                           {32, 8, 0},  {64, 16, 0}, {16, 16, 1}, {64, 64, 0}};
 
 // The row descriptors of M in order `code` (built once per code and cached in M); rays says
 // whether M's rows are rays (A of a ray-traced matrix) or pixels (its transpose).
@@ -276,15 +282,18 @@ const int4 *spmm_order(const fair_csr *h, Csr &M, bool rays, int code) {
     std::vector<int4> d((size_t)nr);
     FAIR_HIP(hipMemcpyAsync(d.data(), M.rows_nz.p, sizeof(int4) * nr, hipMemcpyDeviceToHost, stream()));
     FAIR_HIP(hipStreamSynchronize(stream()));
-    const int64_t N = h->g_n, P = h->g_p;
-    const int t0 = rays ? RAY_TILES[code][0] : PIX_TILES[code][0];
-    const int t1 = rays ? RAY_TILES[code][1] : PIX_TILES[code][1];
-    const int64_t n1 = rays ? (P + t1 - 1) / t1 : (N + t1 - 1) / t1;
+    const int64_t N = h->g_n, P = h->g_p, NTH = h->g_nth;
+    const int *tile = rays ? RAY_TILES[code] : PIX_TILES[code];
+    const int64_t t0 = std::min<int64_t>(tile[0], rays ? NTH : N), t1 = tile[1];
+    const bool band_major = tile[2] != 0;
+    const int64_t n0 = rays ? (NTH + t0 - 1) / t0 : (N + t0 - 1) / t0;   // blocks along u
+    const int64_t n1 = rays ? (P + t1 - 1) / t1 : (N + t1 - 1) / t1;     // blocks along v
     std::vector<std::pair<int64_t, int32_t>> key((size_t)nr);
     for (int64_t s = 0; s < nr; ++s) {
         const int64_t r = d[s].x;
         const int64_t u = rays ? r / P : r / N, v = rays ? r % P : r % N;
-        key[s] = {((u / t0) * n1 + v / t1) * (int64_t)(t0 * t1) + (u % t0) * t1 + v % t1, (int32_t)s};
+        const int64_t blk = band_major ? (v / t1) * n0 + u / t0 : (u / t0) * n1 + v / t1;
+        key[s] = {blk * (t0 * t1) + (u % t0) * t1 + v % t1, (int32_t)s};
     }
     std::sort(key.begin(), key.end());
     std::vector<int4> o((size_t)nr);
@@ -347,10 +356,7 @@ void batch_iterations(fair_csr *h, int method, const float *dB, int32_t k, int32
     const double relax = o ? o->relaxpar : 1.0;
     const BoxF box = make_boxf(o);
     float *cur = dX, *nxt = dXn;
-    const int oa = h->spmm_tile[0], ot = h->spmm_tile[1];
-    // order the rows before a capture starts (the orders are built with synchronous copies)
-    spmm_order(h, A, a_rays, oa);
-    spmm_order(h, T, !a_rays, ot);
+    const int oa = h->spmm_tile[0], ot = h->spmm_tile[1];   // orders built by the caller
     for (int it = 0; it < maxits; ++it) {
         spmm_launch(h, A, a_rays, cur, k, oa, EpiResidualB{dB, M, dR, k}, s);
         EpiUpdateB up{cur, D, nxt, k, relax, box};
@@ -441,6 +447,11 @@ extern "C" int fair_sirt_batch_dev(fair_csr *h, int32_t method, const float *d_B
             if (h->ws_rb.n < need_r) h->ws_rb.alloc(need_r);
             if (h->ws_xb.n < need_x) h->ws_xb.alloc(need_x);
         }
+        // everything a run allocates or looks up is made before a capture can start: the row
+        // orders (cached in the matrices) and the zero row's address
+        spmm_order(h, h->A, !h->transposed, h->spmm_tile[0]);
+        spmm_order(h, h->T, h->transposed, h->spmm_tile[1]);
+        (void)zero_row();
         if (!use_graph) {
             batch_iterations(h, method, d_B, k, maxits, d_X, h->ws_rb.p, h->ws_xb.p, o, stream());
             launch_check("sirt batch");
@@ -561,10 +572,11 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                 FAIR_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
                 FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (dbg)
-                    fprintf(stderr, "[fair] spmm %s order %d (%dx%d): %.1f us\n", dir == 0 ? "A" : "A^T", code,
-                            rays ? RAY_TILES[code][0] : PIX_TILES[code][0],
-                            rays ? RAY_TILES[code][1] : PIX_TILES[code][1], ms * 500.f);
+                if (dbg) {
+                    const int *tl = rays ? RAY_TILES[code] : PIX_TILES[code];
+                    fprintf(stderr, "[fair] spmm %s order %d (%dx%d%s): %.1f us\n", dir == 0 ? "A" : "A^T", code,
+                            tl[0] == ALL ? -1 : tl[0], tl[1], tl[2] ? ", band major" : "", ms * 500.f);
+                }
                 if (ms < best) {
                     best = ms;
                     best_code = code;

@@ -196,7 +196,7 @@ struct fair_csr {
     fair::DBuf<float> ws_r32, ws_x32, ws_rb, ws_xb;   // FP32 / batched workspaces
     bool graphs_used = false;
     // batched SpMM row orders (A, A^T): 0 = stored order, else a sinogram / image tile shape
-    int spmm_tile[2] = {2, 2};
+    int spmm_tile[2] = {3, 6};
     // Forgets the weights and FP32 copies after the matrix changed, and releases the captured
     // batched-SIRT graphs, which hold pointers to them (defined in sparse.hip).
     void drop_cache();

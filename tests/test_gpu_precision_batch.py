@@ -64,3 +64,33 @@ def test_batch_graph_replays_match_eager():
         A.matvecmul(np.ones(A.n))
         g = airms.sirt_batch("cimmino", A, B, [12], options=o, use_graph=True)
         assert g.tobytes() == eager.tobytes(), rep
+
+
+def test_batch_graph_survives_workspace_and_matrix_changes():
+    """Graph replays of fair_sirt_batch_dev after changes to what a captured run refers to: a
+    larger k (the workspaces are reallocated), the earlier k again, and the matrix scaled by
+    multiply_by_scalar (FP32 values and weights rebuilt).  Every graph result equals the eager
+    run of the same call bit for bit."""
+    import torch
+    from fortran_airtools_amd import airms
+    A = c1()
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    B = {k: torch.from_numpy(batch_inputs(A, k).astype(np.float32)).cuda() for k in (8, 16)}
+    # one output panel per k: a graph captured for (B[k], X[k]) is found again by later calls
+    X = {k: torch.zeros((A.n, k), dtype=torch.float32, device="cuda") for k in (8, 16)}
+
+    def run(k, graph):
+        X[k].zero_()
+        torch.cuda.synchronize()
+        airms.sirt_batch_dev("sart", A, B[k].data_ptr(), k, 10, X[k].data_ptr(), options=o, use_graph=graph)
+        torch.cuda.synchronize()
+        return X[k].cpu().numpy()
+
+    for step, k in enumerate((8, 16, 8)):
+        g = run(k, True)
+        assert g.tobytes() == run(k, False).tobytes(), (step, k)
+    first = run(8, True)
+    A.multiply_by_scalar(0.5)
+    g = run(8, True)
+    assert g.tobytes() == run(8, False).tobytes()
+    assert g.tobytes() != first.tobytes()
