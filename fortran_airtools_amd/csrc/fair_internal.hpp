@@ -96,6 +96,31 @@ struct DBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Host array whose elements are not initialised on allocation (a device copy fills them).  Large
+// arrays are 2 MB aligned and marked for transparent huge pages: a device-to-host copy into fresh
+// 4 KB pages spends most of its time on page faults (455 MB: 110 ms at 4 KB pages).
+void *host_alloc_huge(size_t bytes);
+void host_free_huge(void *p);
+
+template <class T>
+struct HostArray {
+    struct Free {
+        void operator()(T *q) const { host_free_huge(q); }
+    };
+    std::unique_ptr<T, Free> p;
+    size_t n = 0;
+    void alloc(size_t count) {
+        p.reset(static_cast<T *>(host_alloc_huge(count * sizeof(T))));
+        n = count;
+    }
+    void release() {
+        p.reset();
+        n = 0;
+    }
+    bool empty() const { return n == 0; }
+    T *data() { return p.get(); }
+};
+
 // ---- device CSR ----------------------------------------------------------------------------
 // Row offsets IA (m+1, int32, zero-based), column indices JA (int32, ZERO-based on the device;
 // the host boundary is one-based), values vA (double).
@@ -167,7 +192,7 @@ struct Csr {
     // Kaczmarz level schedules, least recently used first; shared so that a caller holding one
     // keeps it (and its graph and device buffers) alive when a later plan evicts it
     std::vector<std::shared_ptr<LevelSched>> lev;
-    std::vector<int32_t> hIA, hJA;                  // host copy of the pattern for level planning
+    HostArray<int32_t> hIA, hJA;                    // host copy of the pattern for level planning
                                                     // (kept up to 64 M entries)
 };
 

@@ -10,6 +10,7 @@
 // device: a one-block reduction sets a done flag that gates every later launch, so the whole
 // loop is enqueued without host round trips.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -612,31 +613,74 @@ int env_flag(const char *name, int dflt) {
 // 1 + max(level of the last earlier step touching one of its pixels).  Writes the steps sorted
 // by (level, position) to ordered (S) and the level offsets to off (levels + 1 <= S + 1 entries);
 // returns the number of levels.
-int32_t levelize(int32_t n, const int32_t *IA, const int32_t *JA, int32_t base, const int32_t *seq, int64_t S,
-                 int32_t *ordered, int32_t *off) {
-    std::vector<int32_t> last((size_t)std::max(n, 1), 0), lev((size_t)S);
-    int32_t L = 0;
+//
+// The scan keeps each pixel's last level in `last`, n slots plus a sentinel slot n that columns
+// outside the image read (as 0) and write, so the loops have no branches; 16-bit levels while they
+// fit (half the cache footprint), 32-bit from the start again when they do not.  Built also for
+// AVX2 (8-wide gathers in the max pass) and chosen at run time.
+template <class T>
+__attribute__((always_inline)) inline int64_t level_scan(int32_t n, const int32_t *IA, const int32_t *JA,
+                                                         int32_t base, const int32_t *seq, int64_t S,
+                                                         int32_t *lev, T *last, int64_t cap) {
+    const uint32_t un = (uint32_t)n;
+    int64_t L = 0;
     for (int64_t s = 0; s < S; ++s) {
         const int32_t r = seq[s] - 1;
-        int32_t mx = 0;
-        for (int32_t k = IA[r]; k < IA[r + 1]; ++k) {
-            const int32_t j = JA[k] - base;
-            if ((uint32_t)j < (uint32_t)n && last[j] > mx) mx = last[j];
+        const int32_t b = IA[r], e = IA[r + 1];
+        uint32_t mx = 0;
+        for (int32_t k = b; k < e; ++k) {
+            uint32_t j = (uint32_t)(JA[k] - base);
+            j = j < un ? j : un;
+            const uint32_t v = last[j];
+            mx = v > mx ? v : mx;
         }
         ++mx;
-        for (int32_t k = IA[r]; k < IA[r + 1]; ++k) {
-            const int32_t j = JA[k] - base;
-            if ((uint32_t)j < (uint32_t)n) last[j] = mx;
+        if ((int64_t)mx > cap) return -1;
+        for (int32_t k = b; k < e; ++k) {
+            uint32_t j = (uint32_t)(JA[k] - base);
+            j = j < un ? j : un;
+            last[j] = (T)mx;
         }
-        lev[s] = mx;
-        L = std::max(L, mx);
+        last[un] = 0;
+        lev[s] = (int32_t)mx;
+        L = std::max<int64_t>(L, mx);
     }
+    return L;
+}
+
+template <class T>
+int64_t level_scan_plain(int32_t n, const int32_t *IA, const int32_t *JA, int32_t base, const int32_t *seq,
+                         int64_t S, int32_t *lev, T *last, int64_t cap) {
+    return level_scan<T>(n, IA, JA, base, seq, S, lev, last, cap);
+}
+
+template <class T>
+__attribute__((target("avx2"))) int64_t level_scan_avx2(int32_t n, const int32_t *IA, const int32_t *JA,
+                                                        int32_t base, const int32_t *seq, int64_t S, int32_t *lev,
+                                                        T *last, int64_t cap) {
+    return level_scan<T>(n, IA, JA, base, seq, S, lev, last, cap);
+}
+
+template <class T>
+int64_t level_scan_best(int32_t n, const int32_t *IA, const int32_t *JA, int32_t base, const int32_t *seq,
+                        int64_t S, int32_t *lev, int64_t cap) {
+    std::vector<T> last((size_t)std::max(n, 0) + 1, 0);
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    return avx2 ? level_scan_avx2<T>(n, IA, JA, base, seq, S, lev, last.data(), cap)
+                : level_scan_plain<T>(n, IA, JA, base, seq, S, lev, last.data(), cap);
+}
+
+int32_t levelize(int32_t n, const int32_t *IA, const int32_t *JA, int32_t base, const int32_t *seq, int64_t S,
+                 int32_t *ordered, int32_t *off) {
+    std::vector<int32_t> lev((size_t)S);
+    int64_t L = level_scan_best<uint16_t>(n, IA, JA, base, seq, S, lev.data(), 65535);
+    if (L < 0) L = level_scan_best<int32_t>(n, IA, JA, base, seq, S, lev.data(), INT32_MAX);
     std::fill(off, off + L + 1, 0);
     for (int64_t s = 0; s < S; ++s) ++off[lev[s]];
-    for (int32_t l = 0; l < L; ++l) off[l + 1] += off[l];
+    for (int64_t l = 0; l < L; ++l) off[l + 1] += off[l];
     std::vector<int32_t> fill(off, off + L);
     for (int64_t s = 0; s < S; ++s) ordered[fill[lev[s] - 1]++] = seq[s];
-    return L;
+    return (int32_t)L;
 }
 
 // The level schedule of a row sequence (one-based rows), made on the host from the pattern and
@@ -656,21 +700,25 @@ std::shared_ptr<LevelSched> art_levels(Csr &A, const int32_t *seq, int64_t S) {
             return S >= width * c->nlev ? c : nullptr;
         }
     }
+    const bool dbg = env_flag("FAIR_DEBUG", 0) != 0;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    const auto t0 = now();
     if (A.hIA.empty()) {
-        A.hIA.resize((size_t)A.m + 1);
-        A.hJA.resize((size_t)std::max<int64_t>(A.nnz, 1));
+        A.hIA.alloc((size_t)A.m + 1);
+        A.hJA.alloc((size_t)std::max<int64_t>(A.nnz, 1));
         A.IA.download(A.hIA.data(), (size_t)A.m + 1);
         A.JA.download(A.hJA.data(), (size_t)A.nnz);
         FAIR_HIP(hipStreamSynchronize(stream()));
     }
+    const auto t1 = now();
     // the host copy is kept for later plans up to 64 M entries (256 MB) and released after this
     // plan above that
     struct Release {
         Csr &A;
         ~Release() {
             if (A.nnz > (int64_t(1) << 26)) {
-                std::vector<int32_t>().swap(A.hIA);
-                std::vector<int32_t>().swap(A.hJA);
+                A.hIA.release();
+                A.hJA.release();
             }
         }
     } release{A};
@@ -687,6 +735,7 @@ std::shared_ptr<LevelSched> art_levels(Csr &A, const int32_t *seq, int64_t S) {
     auto c = std::make_shared<LevelSched>();
     c->seq.assign(seq, seq + S);
     const int32_t L = levelize(A.n, IA, JA, 0, seq, S, ordered.data(), off.data());
+    const auto t2 = now();
     off.resize((size_t)L + 1);
     c->nlev = L;
     c->rows.alloc((size_t)S);
@@ -705,6 +754,11 @@ std::shared_ptr<LevelSched> art_levels(Csr &A, const int32_t *seq, int64_t S) {
     c->off.upload(off.data(), (size_t)L + 1);
     FAIR_HIP(hipMemcpyAsync(c->desc.p, desc.data(), sizeof(int4) * S, hipMemcpyHostToDevice, stream()));
     FAIR_HIP(hipStreamSynchronize(stream()));
+    if (dbg) {
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        fprintf(stderr, "[fair] level plan of %lld steps: pattern copy %.1f ms, levels %.1f ms, descriptors %.1f ms\n",
+                (long long)S, ms(t0, t1), ms(t1, t2), ms(t2, now()));
+    }
     c->hoff = std::move(off);
     if (A.lev.size() >= 6) A.lev.erase(A.lev.begin());
     A.lev.push_back(c);
@@ -718,6 +772,7 @@ std::shared_ptr<LevelSched> art_levels(Csr &A, const int32_t *seq, int64_t S) {
 void graph_art_levels(const Csr &A, LevelSched *s, double *x, const double *b, const double *nrA, double relax,
                       const Box &box, hipStream_t st) {
     if (!s->graph) {
+        const auto tc = std::chrono::steady_clock::now();
         if (!g_level_capture) FAIR_HIP(hipStreamCreateWithFlags(&g_level_capture, hipStreamNonBlocking));
         s->params.alloc(1);
         FAIR_HIP(hipStreamSynchronize(stream()));
@@ -742,6 +797,9 @@ void graph_art_levels(const Csr &A, LevelSched *s, double *x, const double *b, c
         (void)hipGraphDestroy(g);
         FAIR_HIP(ie);
         s->graph = ex;
+        if (env_flag("FAIR_DEBUG", 0))
+            fprintf(stderr, "[fair] level graph of %d launches captured in %.1f ms\n", s->nlev,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count());
     }
     hipLaunchKernelGGL(k_set_art_params, dim3(1), dim3(1), 0, st, s->params.p,
                        ArtParams{x, b, nrA, relax, box.lb, box.ub, box.lwrb, box.upb});

@@ -9,6 +9,8 @@
 // one load (for odd detector counts the fan geometry leaves about half the rows empty).
 // A^T*y is a gather over a device-built CSR of A^T instead of the reference's racy scatter:
 // deterministic, and every thread owns its output.
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -39,6 +41,22 @@ void before_device_alloc(size_t bytes) {
         throw Error{FAIR_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + " bytes): out of memory (allocation " +
                                       std::to_string(k) + " failed by FAIR_FAIL_ALLOC)"};
 }
+void *host_alloc_huge(size_t bytes) {
+    constexpr size_t HUGE_PAGE = size_t(2) << 20;
+    void *q = nullptr;
+    if (bytes >= HUGE_PAGE) {
+        const size_t rounded = (bytes + HUGE_PAGE - 1) / HUGE_PAGE * HUGE_PAGE;
+        if (posix_memalign(&q, HUGE_PAGE, rounded) != 0) q = nullptr;
+        if (q) (void)madvise(q, rounded, MADV_HUGEPAGE);   // advice only: ignored where unsupported
+    } else {
+        q = malloc(bytes > 0 ? bytes : 1);
+    }
+    if (!q) throw std::bad_alloc();
+    return q;
+}
+
+void host_free_huge(void *p) { free(p); }
+
 hipStream_t stream() { return g_stream; }
 int lds_per_block() { return g_lds_per_block; }
 
