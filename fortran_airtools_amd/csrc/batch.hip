@@ -305,6 +305,168 @@ const int4 *spmm_order(const fair_csr *h, Csr &M, bool rays, int code) {
     return (const int4 *)M.spmm_rows.p;
 }
 
+// ---- batched: tiled SpMM for rows that share their columns (pixels of A^T) ------------------------
+// A block of up to 16 wavefronts takes the rows of one tile (pixels of a small image tile, which are
+// crossed by largely the same rays).  The union of the tile's columns, sorted, is staged through LDS
+// in chunks of STILE_CHUNK panel rows; each wavefront walks its own row's entries (ascending, so
+// every chunk's entries of a row are contiguous) and reads the gathered rows from LDS.  A union
+// column is fetched from L2 once per tile instead of once per entry.  Each lane sums its row's
+// entries in the stored order with the same expression as k_spmm_rows, so both kernels give the
+// same bits.  One pass over k <= 256 right-hand sides (k a multiple of 4).
+constexpr int STILE_ROWS = 16;
+constexpr int STILE_CHUNK = 64;
+constexpr int STILE_SHAPES = 5;
+constexpr int STILE[STILE_SHAPES][2] = {{4, 4}, {8, 2}, {2, 8}, {16, 1}, {1, 16}};
+constexpr int STILE_CODE0 = 100;   // order codes STILE_CODE0 + i select the tiled kernel, shape i
+
+template <class Epi>
+__global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ binfo, int32_t nblocks, int32_t per_xcd,
+                                                     const int4 *__restrict__ rows, const int32_t *__restrict__ uni,
+                                                     const uint16_t *__restrict__ slot, const float *__restrict__ vA,
+                                                     const float *__restrict__ X, int32_t k, Epi epi) {
+    extern __shared__ float4 stage[];   // STILE_CHUNK rows of k / 4 float4
+    const int b = blockIdx.x;
+    const int lb = (b % NXCD) * per_xcd + b / NXCD;
+    if (lb >= nblocks) return;          // the whole block leaves together
+    const int4 bi = binfo[lb];
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & (WAVE - 1));
+    const int kv = k >> 2;
+    const bool has = w < bi.y;
+    const int4 ri = has ? rows[bi.x + w] : make_int4(0, 0, 0, 0);
+    int q = ri.y;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    // the next chunk's panel rows are loaded into registers while the current chunk is used
+    constexpr int PER_WAVE = STILE_CHUNK / STILE_ROWS;
+    float4 pre[PER_WAVE];
+    auto fetch = [&](int c0) {
+        const int cn = min(STILE_CHUNK, bi.w - c0);
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int u = w + STILE_ROWS * i;
+            pre[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (u < cn && lane < kv) {
+                const int col = uni[bi.z + c0 + u];
+                pre[i] = reinterpret_cast<const float4 *>(X + (int64_t)col * k)[lane];
+            }
+        }
+    };
+    fetch(0);
+    for (int c0 = 0; c0 < bi.w; c0 += STILE_CHUNK) {
+        const int cn = min(STILE_CHUNK, bi.w - c0);
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int u = w + STILE_ROWS * i;
+            if (u < cn && lane < kv) stage[u * kv + lane] = pre[i];
+        }
+        __syncthreads();
+        if (c0 + STILE_CHUNK < bi.w) fetch(c0 + STILE_CHUNK);
+        if (has) {
+            for (;;) {
+                const int qq = q + lane;
+                const int sl = qq < ri.z ? (int)slot[qq] : 1 << 20;
+                const float vv = qq < ri.z ? vA[qq] : 0.f;
+                const uint64_t inmask = __ballot(sl < c0 + cn);   // a prefix: the slots ascend
+                const int nin = __popcll(inmask);
+                for (int t = 0; t < nin; ++t) {
+                    const int su = __builtin_amdgcn_readlane(sl, t) - c0;
+                    const float wv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vv), t));
+                    if (lane < kv) {
+                        const float4 xv = stage[su * kv + lane];
+                        acc[0] += wv * xv.x;
+                        acc[1] += wv * xv.y;
+                        acc[2] += wv * xv.z;
+                        acc[3] += wv * xv.w;
+                    }
+                }
+                q += nin;
+                if (nin < WAVE) break;   // the rest of the row lies in later chunks
+            }
+        }
+        __syncthreads();
+    }
+    if (has && lane < kv) epi.template apply<4>(ri.x, lane * 4, acc);
+}
+
+// The tiled layout of M (rows = pixels, j = x * N + y, columns = rays) for tile shape i: built on
+// the host from M's pattern once per shape and cached in M; false when it does not apply (a tile's
+// union exceeds 65,535 columns, or a column lies outside the matrix).
+bool build_stiles(const fair_csr *h, Csr &M, int shape) {
+    if (M.stile.code == shape) return M.stile.nblocks > 0;
+    M.stile.code = shape;
+    M.stile.nblocks = 0;
+    const int64_t N = h->g_n;
+    if (N <= 0 || (int64_t)M.m != N * N || M.nnz <= 0) return false;
+    std::vector<int32_t> IA((size_t)M.m + 1), JA((size_t)M.nnz);
+    M.IA.download(IA.data(), (size_t)M.m + 1);
+    M.JA.download(JA.data(), (size_t)M.nnz);
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    const int TX = STILE[shape][0], TY = STILE[shape][1];
+    std::vector<int4> binfo, rows;
+    std::vector<int32_t> uni;
+    std::vector<uint16_t> slot((size_t)M.nnz);
+    std::vector<int32_t> mark((size_t)std::max(M.n, 1), -1), pos((size_t)std::max(M.n, 1), 0);
+    std::vector<int32_t> u;
+    int32_t tile = 0;
+    for (int64_t x0 = 0; x0 < N; x0 += TX)
+        for (int64_t y0 = 0; y0 < N; y0 += TY, ++tile) {
+            const int32_t rb = (int32_t)rows.size();
+            u.clear();
+            for (int64_t x = x0; x < std::min(N, x0 + TX); ++x)
+                for (int64_t y = y0; y < std::min(N, y0 + TY); ++y) {
+                    const int64_t j = x * N + y;
+                    if (IA[j + 1] == IA[j]) continue;
+                    rows.push_back(make_int4((int32_t)j, IA[j], IA[j + 1], 0));
+                    for (int32_t e = IA[j]; e < IA[j + 1]; ++e) {
+                        const int32_t c = JA[e];
+                        if ((uint32_t)c >= (uint32_t)M.n) return false;
+                        if (mark[c] != tile) {
+                            mark[c] = tile;
+                            u.push_back(c);
+                        }
+                    }
+                }
+            const int32_t nr = (int32_t)rows.size() - rb;
+            if (nr == 0) continue;
+            if (u.size() > 65535) return false;
+            std::sort(u.begin(), u.end());
+            for (size_t i = 0; i < u.size(); ++i) pos[u[i]] = (int32_t)i;
+            for (int32_t r = rb; r < rb + nr; ++r)
+                for (int32_t e = rows[r].y; e < rows[r].z; ++e) slot[e] = (uint16_t)pos[JA[e]];
+            binfo.push_back(make_int4(rb, nr, (int32_t)uni.size(), (int32_t)u.size()));
+            uni.insert(uni.end(), u.begin(), u.end());
+        }
+    auto up = [](DBuf<int32_t> &d, const void *src, size_t ints) {
+        d.alloc(ints > 0 ? ints : 1);
+        if (ints) FAIR_HIP(hipMemcpyAsync(d.p, src, ints * sizeof(int32_t), hipMemcpyHostToDevice, stream()));
+    };
+    up(M.stile.binfo, binfo.data(), 4 * binfo.size());
+    up(M.stile.rows, rows.data(), 4 * rows.size());
+    up(M.stile.uni, uni.data(), uni.size());
+    M.stile.slot.alloc((size_t)M.nnz);
+    M.stile.slot.upload(slot.data(), (size_t)M.nnz);
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    M.stile.nblocks = (int32_t)binfo.size();
+    return M.stile.nblocks > 0;
+}
+
+// Whether order code `code` selects the tiled kernel for M with k right-hand sides (and its layout
+// is built); pixel rows only.
+bool stiles_apply(const fair_csr *h, Csr &M, bool rays, int code, int32_t k) {
+    if (rays || code < STILE_CODE0 || code >= STILE_CODE0 + STILE_SHAPES || k % 4 != 0 || k > 4 * WAVE) return false;
+    if ((size_t)STILE_CHUNK * (size_t)k * sizeof(float) > (size_t)lds_per_block()) return false;
+    return build_stiles(h, M, code - STILE_CODE0);
+}
+
+template <class Epi>
+void spmm_launch_tiles(const Csr &M, const float *X, int32_t k, Epi epi, hipStream_t s) {
+    const int64_t blocks = M.stile.nblocks;
+    const int64_t per_xcd = (blocks + NXCD - 1) / NXCD;
+    const size_t lds = (size_t)STILE_CHUNK * (size_t)k * sizeof(float);
+    hipLaunchKernelGGL((k_spmm_tiles<Epi>), dim3((unsigned)(per_xcd * NXCD)), dim3(64 * STILE_ROWS), lds, s,
+                       (const int4 *)M.stile.binfo.p, (int32_t)blocks, (int32_t)per_xcd, (const int4 *)M.stile.rows.p,
+                       M.stile.uni.p, M.stile.slot.p, M.vA32.p, X, k, epi);
+}
+
 template <int VEC, class Epi>
 void spmm_launch_v(const int4 *rows, int32_t nrows, const Csr &M, const float *X, int32_t k, Epi epi,
                    hipStream_t s) {
@@ -314,12 +476,24 @@ void spmm_launch_v(const int4 *rows, int32_t nrows, const Csr &M, const float *X
                        (int32_t)per_xcd, M.JA.p, M.vA32.p, M.n, X, k, zero_row(), epi);
 }
 
+// Everything the products of a batched run build lazily (row orders, tiled layouts), made ahead.
+void spmm_prepare(const fair_csr *h, int32_t k) {
+    Csr &A = const_cast<fair_csr *>(h)->A, &T = const_cast<fair_csr *>(h)->T;
+    const bool a_rays = !h->transposed;
+    if (!stiles_apply(h, A, a_rays, h->spmm_tile[0], k)) spmm_order(h, A, a_rays, h->spmm_tile[0]);
+    if (!stiles_apply(h, T, !a_rays, h->spmm_tile[1], k)) spmm_order(h, T, !a_rays, h->spmm_tile[1]);
+}
+
 // Y = M * X (FP32) through Epi, rows of M in order `code`.
 template <class Epi>
 void spmm_launch(const fair_csr *h, Csr &M, bool rays, const float *X, int32_t k, int code, Epi epi,
                  hipStream_t s) {
     if (M.n_rows_nz <= 0 || k <= 0) return;
-    const int4 *rows = spmm_order(h, M, rays, code);
+    if (stiles_apply(h, M, rays, code, k)) {
+        spmm_launch_tiles(M, X, k, epi, s);
+        return;
+    }
+    const int4 *rows = spmm_order(h, M, rays, code >= STILE_CODE0 ? 0 : code);
     if (k % 4 == 0) spmm_launch_v<4>(rows, M.n_rows_nz, M, X, k, epi, s);
     else spmm_launch_v<1>(rows, M.n_rows_nz, M, X, k, epi, s);
 }
@@ -447,10 +621,17 @@ extern "C" int fair_sirt_batch_dev(fair_csr *h, int32_t method, const float *d_B
             if (h->ws_rb.n < need_r) h->ws_rb.alloc(need_r);
             if (h->ws_xb.n < need_x) h->ws_xb.alloc(need_x);
         }
+        // FAIR_SPMM_ORDERS=a,t fixes the two products' row orders (tests and experiments)
+        if (const char *e = getenv("FAIR_SPMM_ORDERS")) {
+            int oa = 0, ot = 0;
+            if (sscanf(e, "%d,%d", &oa, &ot) == 2) {
+                h->spmm_tile[0] = oa;
+                h->spmm_tile[1] = ot;
+            }
+        }
         // everything a run allocates or looks up is made before a capture can start: the row
-        // orders (cached in the matrices) and the zero row's address
-        spmm_order(h, h->A, !h->transposed, h->spmm_tile[0]);
-        spmm_order(h, h->T, h->transposed, h->spmm_tile[1]);
+        // orders and tiled layouts (cached in the matrices) and the zero row's address
+        spmm_prepare(h, k);
         (void)zero_row();
         if (!use_graph) {
             batch_iterations(h, method, d_B, k, maxits, d_X, h->ws_rb.p, h->ws_xb.p, o, stream());
@@ -562,7 +743,12 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
             const float *src = dir == 0 ? X.p : R.p;
             float best = 1e30f;
             int best_code = 0;
-            for (int code = 0; code < (geo ? SPMM_ORDERS : 1); ++code) {
+            std::vector<int> codes;
+            for (int c = 0; c < (geo ? SPMM_ORDERS : 1); ++c) codes.push_back(c);
+            for (int c = 0; c < STILE_SHAPES; ++c)
+                if (geo && !rays && k % 4 == 0 && k <= 4 * WAVE) codes.push_back(STILE_CODE0 + c);
+            for (int code : codes) {
+                if (code >= STILE_CODE0 && !stiles_apply(h, M, rays, code, k)) continue;
                 auto run = [&] { spmm_launch(h, M, rays, src, k, code, EpiStoreB{Y.p, k}, stream()); };
                 run();
                 FAIR_HIP(hipEventRecord(e0, stream()));
@@ -572,7 +758,11 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                 FAIR_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
                 FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (dbg) {
+                if (dbg && code >= STILE_CODE0) {
+                    fprintf(stderr, "[fair] spmm %s tiled %d (%dx%d pixels, LDS-staged union): %.1f us\n",
+                            dir == 0 ? "A" : "A^T", code, STILE[code - STILE_CODE0][0], STILE[code - STILE_CODE0][1],
+                            ms * 500.f);
+                } else if (dbg) {
                     const int *tl = rays ? RAY_TILES[code] : PIX_TILES[code];
                     fprintf(stderr, "[fair] spmm %s order %d (%dx%d%s): %.1f us\n", dir == 0 ? "A" : "A^T", code,
                             tl[0] == ALL ? -1 : tl[0], tl[1], tl[2] ? ", band major" : "", ms * 500.f);
@@ -583,7 +773,7 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                 }
             }
             h->spmm_tile[dir] = best_code;
-            spmm_order(h, M, rays, best_code);
+            if (!stiles_apply(h, M, rays, best_code, k)) spmm_order(h, M, rays, best_code);
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

@@ -189,6 +189,15 @@ struct Csr {
     Blocked blk;             // LDS-staged layout (built for ray-traced matrices)
     DBuf<int32_t> spmm_rows; // rows_nz in the batched SpMM's tile order spmm_code (int4 each)
     int32_t spmm_code = -1;
+    // Tiled layout of the batched SpMM (rows grouped in tiles; each tile's union of columns,
+    // sorted, is staged through LDS in chunks): built for tile shape stile_code
+    struct {
+        int32_t code = -1, nblocks = 0;
+        DBuf<int32_t> binfo;   // int4 per tile {row_begin, nrows, union_begin, union_len}
+        DBuf<int32_t> rows;    // int4 per row {row, start, end, 0}
+        DBuf<int32_t> uni;     // the tiles' union columns
+        DBuf<uint16_t> slot;   // per entry: its column's position in its tile's union
+    } stile;
     // Kaczmarz level schedules, least recently used first; shared so that a caller holding one
     // keeps it (and its graph and device buffers) alive when a later plan evicts it
     std::vector<std::shared_ptr<LevelSched>> lev;

@@ -313,6 +313,8 @@ void build_row_list(Csr &A) {
     A.max_row = hmx;
     A.spmm_rows.release();   // the batched SpMM's reordered copy of the list
     A.spmm_code = -1;
+    A.stile.code = -1;       // and its tiled layout
+    A.stile.nblocks = 0;
 }
 
 // Sort the entries of every listed row by column index (values follow).

@@ -94,3 +94,25 @@ def test_batch_graph_survives_workspace_and_matrix_changes():
     g = run(8, True)
     assert g.tobytes() == run(8, False).tobytes()
     assert g.tobytes() != first.tobytes()
+
+
+@pytest.mark.parametrize("tile_code", [100, 101, 102, 103, 104])
+def test_tiled_transpose_product_equals_row_kernel(monkeypatch, tile_code):
+    """The LDS-staged tiled kernel of A^T*R (pixels of an image tile, their rays' union staged in
+    chunks) gives the same bits as the wavefront-per-row kernel: every pixel's sum runs over its
+    entries in the stored order with the same operations.  k = 32 and k = 256 right-hand sides."""
+    import torch
+    from fortran_airtools_amd import airms
+    A = c1()
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    for k in (32, 256):
+        B = torch.from_numpy(batch_inputs(A, k).astype(np.float32)).cuda()
+        out = {}
+        for codes in ("3,6", f"3,{tile_code}"):
+            monkeypatch.setenv("FAIR_SPMM_ORDERS", codes)
+            X = torch.zeros((A.n, k), dtype=torch.float32, device="cuda")
+            torch.cuda.synchronize()
+            airms.sirt_batch_dev("sart", A, B.data_ptr(), k, 12, X.data_ptr(), options=o)
+            torch.cuda.synchronize()
+            out[codes] = X.cpu().numpy()
+        assert out["3,6"].tobytes() == out[f"3,{tile_code}"].tobytes(), k
