@@ -27,7 +27,7 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-
 # forward and adjoint kernels evaluate the ray geometry with identical roundings (fused
 # multiply-adds are written out where wanted).
 PER_FILE = {"raytrace.hip": ["-ffp-contract=off"], "sparse.hip": ["-ffp-contract=off"],
-            "methods.hip": ["-ffp-contract=off"], "group.hip": ["-ffp-contract=off"],
+            "methods.hip": ["-ffp-contract=off"], "artwin.hip": ["-ffp-contract=off"], "group.hip": ["-ffp-contract=off"],
             "matfree.hip": ["-ffp-contract=off"], "phantoms.hip": ["-ffp-contract=off"]}
 
 

@@ -1,0 +1,455 @@
+// artwin.hip -- Kaczmarz row sequences without a level schedule (random orders, row-norm
+// sampling, a caller's sequence), walked a window of W steps at a time by one workgroup.
+//
+// basic_sweep (AIRMs.f90:813-858) is a chain: each row step reads the pixels the earlier steps
+// wrote.  A one-wavefront sweep pays a global-memory round trip and the whole row's instruction
+// stream per step.  Here the work of a window that does not depend on the order is done side by
+// side: every step's entries and x values are loaded at the window's start, and only the values
+// of pixels that two or more steps of the window touch ("shared" pixels) pass from step to step,
+// through LDS.  A step waits only until every earlier step of its window up to the last one that
+// shares a pixel with it is done, so steps that do not conflict run at the same time.
+//
+// A pre-pass kernel classifies each entry of each step (per window, from a bit mask of the steps
+// touching each pixel):
+//   0xFFFF                     the pixel is touched by no other step of the window: the step
+//                              uses the window-start value and stores its result to x;
+//   slot | P << 14 | L << 15   shared: P = an earlier step of the window touched it (read the
+//                              value from LDS slot `slot`, else use the window-start value),
+//                              L = no later step touches it (store the result to x);
+// and records per step dep = the latest earlier step of its window sharing a pixel (-1: none).
+// The executor then computes every row exactly as k_art_sweep does (entries of lane q:
+// beg + q + 64 t in increasing t, products and sums rounded separately, the DPP wavefront sum,
+// coeff = relax*(b - dot)/nrA, x + coeff*a then max(lb), min(ub)), on the same values in the
+// same order, so its iterate equals k_art_sweep's bit for bit.
+#include <algorithm>
+#include <cstdlib>
+
+#include "fair_internal.hpp"
+
+namespace fair {
+namespace {
+
+constexpr uint16_t WIN_UNSHARED = 0xFFFF;
+constexpr int WIN_SLOT_MASK = 0x1FFF;
+constexpr int WIN_HASPRED = 1 << 14;
+constexpr int WIN_LAST = 1 << 15;
+constexpr int WIN_PRE_WAVES = 8;   // pre-pass workgroup: 512 threads
+
+// Window sizes: W steps, NW wavefronts of SPW steps each (wavefront v: steps v, v + NW, ...).
+// Rows up to 64 R entries.  LDS holds a double per shared pixel: at most half of a window's
+// entries, W * 64 R / 2 (64 KB for both shapes).
+template <int R> struct WinShape;
+template <> struct WinShape<8> { static constexpr int NW = 8, SPW = 3; };
+template <> struct WinShape<16> { static constexpr int NW = 8, SPW = 1; };
+
+template <int R> constexpr int win_steps() { return WinShape<R>::NW * WinShape<R>::SPW; }
+template <int R> constexpr int win_pub() { return win_steps<R>() * WAVE * R / 2; }
+
+struct WinChains {
+    int64_t nsteps[2];
+    double *x[2];
+    const int4 *desc[2];        // per step {row (zero-based), IA(row), IA(row+1), dep}
+    const double2 *bw[2];       // per step {b(row), nrA(row)}
+    const uint16_t *codes[2];   // 64 R per step, lane-major (the R codes of a lane are adjacent)
+};
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- pre-pass: entry codes and step descriptors of one chain -------------------------------------
+// Each workgroup takes windows blockIdx.x, + gridDim.x, ... and owns the tables mask / slotof
+// (n entries each) at offset blockIdx.x * n; mask is all zero between windows (every window
+// clears the entries it set).  Within a window:
+//   A  every entry ORs its step's bit into mask[pixel];
+//   B  every entry reads the mask; the first step of a shared pixel takes an LDS slot number
+//      and publishes it in slotof[pixel]; each step's dep is the largest earlier bit, written
+//      with the step's row, entry range, b and nrA;
+//   C  every entry writes its code;
+//   D  every entry clears mask[pixel].
+// The tables are accessed with device-scope atomics, and a fence and barrier separate the
+// phases.
+template <int R>
+__global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_t *__restrict__ rows,
+                                                                    int64_t nsteps,
+                                                                    const int32_t *__restrict__ IA,
+                                                                    const int32_t *__restrict__ JA,
+                                                                    const double *__restrict__ b,
+                                                                    const double *__restrict__ nrA,
+                                                                    int32_t ncols, uint32_t *mask_all,
+                                                                    int32_t *slot_all, uint16_t *codes,
+                                                                    int4 *desc, double2 *bw) {
+    constexpr int W = win_steps<R>();
+    constexpr int SPW = W / WIN_PRE_WAVES;
+    static_assert(W % WIN_PRE_WAVES == 0 && W <= 32, "window shape");
+    __shared__ int32_t nslot;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint32_t *mask = mask_all + (size_t)blockIdx.x * (size_t)ncols;
+    int32_t *slotof = slot_all + (size_t)blockIdx.x * (size_t)ncols;
+    if (threadIdx.x == 0) nslot = 0;
+    __syncthreads();
+    const int64_t nwin = (nsteps + W - 1) / W;
+    for (int64_t k = blockIdx.x; k < nwin; k += gridDim.x) {
+        int32_t jc[SPW][R];
+        uint32_t mk[SPW][R];
+        int32_t rq[SPW], bq[SPW], eq[SPW];
+#pragma unroll
+        for (int q = 0; q < SPW; ++q) {
+            const int i = wv + WIN_PRE_WAVES * q;
+            const int64_t s = k * W + i;
+            const bool ok = s < nsteps;
+            const int32_t r = ok ? rows[s] - 1 : 0;
+            const int32_t beg = IA[r], end = ok ? IA[r + 1] : beg;
+            rq[q] = r;
+            bq[q] = beg;
+            eq[q] = end;
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int32_t e = beg + lane + WAVE * u;
+                const int32_t j = e < end ? JA[e] : -1;
+                jc[q][u] = (uint32_t)j < (uint32_t)ncols ? j : -1;
+                if (jc[q][u] >= 0)
+                    (void)__hip_atomic_fetch_or(&mask[jc[q][u]], 1u << i, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __threadfence();
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < SPW; ++q) {
+            const int i = wv + WIN_PRE_WAVES * q;
+            int d = -1;
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int32_t j = jc[q][u];
+                const uint32_t M = j >= 0 ? ld_agent(&mask[j]) : 0u;
+                mk[q][u] = M;
+                if (__popc(M) >= 2) {
+                    if ((M & (0u - M)) == (1u << i)) {
+                        const int32_t sl = atomicAdd(&nslot, 1);
+                        __hip_atomic_store(&slotof[j], sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    const uint32_t below = M & ((1u << i) - 1u);
+                    if (below) d = max(d, 31 - __clz(below));
+                }
+            }
+#pragma unroll
+            for (int off = WAVE / 2; off > 0; off >>= 1) d = max(d, __shfl_xor(d, off, WAVE));
+            const int64_t s = k * W + i;
+            if (lane == 0 && s < nsteps) {
+                desc[s] = make_int4(rq[q], bq[q], eq[q], d);
+                bw[s] = make_double2(b[rq[q]], nrA[rq[q]]);
+            }
+        }
+        __threadfence();
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < SPW; ++q) {
+            const int i = wv + WIN_PRE_WAVES * q;
+            const int64_t s = k * W + i;
+            if (s >= nsteps) continue;
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int32_t j = jc[q][u];
+                const uint32_t M = mk[q][u];
+                uint16_t c = WIN_UNSHARED;
+                if (__popc(M) >= 2) {
+                    const int32_t sl = __hip_atomic_load(&slotof[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    c = (uint16_t)((sl & WIN_SLOT_MASK) | ((M & ((1u << i) - 1u)) ? WIN_HASPRED : 0) |
+                                   ((M >> i) == 1u ? WIN_LAST : 0));
+                }
+                codes[s * (WAVE * R) + lane * R + u] = c;
+            }
+        }
+        __syncthreads();   // every slot number is read; nslot is no longer used by this window
+#pragma unroll
+        for (int q = 0; q < SPW; ++q)
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+                if (jc[q][u] >= 0)
+                    __hip_atomic_store(&mask[jc[q][u]], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) nslot = 0;
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+// wave_sum_dpp's sum (the same lane pairs in the same order, so the same bits) with DPP moves
+// that need no zero-filled destination.
+template <int CTRL>
+__device__ __forceinline__ double win_dpp(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double win_wave_sum(double v) {
+    v += win_dpp<0xB1>(v);
+    v += win_dpp<0x4E>(v);
+    v += win_dpp<0x141>(v);
+    v += win_dpp<0x140>(v);
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
+// The value a padded slot gathers (its product must be +0 whatever x holds).
+__device__ double g_win_zero = 0.0;
+
+__device__ __forceinline__ bool win_shared(int32_t c) { return c != WIN_UNSHARED; }
+__device__ __forceinline__ bool win_haspred(int32_t c) { return c != WIN_UNSHARED && (c & WIN_HASPRED); }
+
+// ---- executor ----------------------------------------------------------------------------------
+// One workgroup per chain.  Each wavefront holds the entries and codes of its SPW steps of the
+// current window, and the descriptors of those steps in this window and the next (one lane per
+// step); right after its turn a step loads its next-window entries, so at a window's start only
+// the x values are loaded (the barrier that ends a window orders its stores before them).
+// A turn: wait until done[l] == window for every l <= dep (LDS flags, one per step of the window);
+// replace the values of the pixels an earlier step of the window wrote by theirs from LDS; compute
+// the row; write the new values of its shared pixels to LDS and mark itself done; then store the
+// unshared pixels (and the shared ones no later step touches) to x.  Every slot is computed as
+// k_art_sweep computes it (slots past the row hold zero products and are not stored), without
+// branches, so the compiler can overlap the slots.  While a step computes its row its wavefront
+// runs at a raised issue priority, ahead of the wavefronts that poll.
+// Waits are bounded: after spin_limit polls a wavefront sets the stop flag and *err and stops
+// computing, the other wavefronts stop at their next poll, and every wavefront still passes the
+// same barriers, so the launch ends normally; a bound of 0 fails the first wait (failure-path
+// tests).
+template <int R>
+__global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains ch, const int32_t *__restrict__ JA,
+                                                                     const double *__restrict__ vA, int32_t pad,
+                                                                     double relax, int lwrb, double lb, int upb,
+                                                                     double ub, int32_t spin_limit, int32_t *err) {
+    constexpr int NW = WinShape<R>::NW, SPW = WinShape<R>::SPW, W = NW * SPW, PUB = win_pub<R>();
+    __shared__ double pub[PUB];
+    __shared__ int32_t done[W + 1];   // done[W]: the stop flag (0 while running)
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t nsteps = ch.nsteps[blockIdx.x];
+    double *x = ch.x[blockIdx.x];
+    const int4 *__restrict__ desc = ch.desc[blockIdx.x];
+    const double2 *__restrict__ bw = ch.bw[blockIdx.x];
+    const uint16_t *__restrict__ codes = ch.codes[blockIdx.x];
+    if (threadIdx.x <= W) done[threadIdx.x] = threadIdx.x < W ? -1 : 0;
+    __syncthreads();
+    const int64_t nwin = (nsteps + W - 1) / W;
+    if (nwin == 0) return;   // the whole workgroup leaves together
+    int32_t jc[SPW][R];
+    uint32_t cw[SPW][R / 2];   // the lane's codes, two per word
+    double vc[SPW][R], xv[SPW][R];
+    // Descriptors of the wavefront's steps, lane q holding step q: {beg, end, dep} and {b, nrA} of
+    // this window (cur) and of the next (nxt), loaded a window ahead by one vector load each.
+    int4 cur_d, nxt_d;
+    double2 cur_b, nxt_b;
+    bool bailed = false;
+    // the box bounds live in vector registers: a select then needs no scalar-to-vector move
+    double lbv = lb, ubv = ub;
+    asm volatile("" : "+v"(lbv), "+v"(ubv));
+    auto code = [&](int q, int u) -> int32_t { return (int32_t)((cw[q][u >> 1] >> ((u & 1) * 16)) & 0xFFFFu); };
+    // lanes 0..SPW-1: the descriptors of the wavefront's steps of window k (a step past the end: no
+    // entries, no wait)
+    auto load_descs = [&](int64_t k, int4 &d, double2 &bv) {
+        const int64_t s = k * W + wv + NW * (lane < SPW ? lane : 0);
+        const bool ok = s < nsteps;
+        const int4 t = desc[ok ? s : 0];
+        d = ok ? t : make_int4(t.x, t.y, t.y, -1);
+        bv = bw[ok ? s : 0];
+    };
+    // entries and codes of step q of window k, entry range [beg, end).  Slots past the row load
+    // the zero spare entry `pad` after the matrix's entries (column 0, value 0): no load result is
+    // selected afterwards, so the loads stay in flight together; a slot's validity is tested
+    // where it matters (lane + 64 u < the row's length).
+    auto load_entries = [&](int q, int64_t k, int32_t beg, int32_t end) {
+        const int64_t s = k * W + wv + NW * q;
+        const int64_t sc = s < nsteps ? s : 0;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int32_t e = beg + lane + WAVE * u;
+            const int32_t ee = e < end ? e : pad;
+            jc[q][u] = JA[ee];
+            vc[q][u] = vA[ee];
+        }
+        const uint4 *cp = reinterpret_cast<const uint4 *>(codes + sc * (WAVE * R) + lane * R);
+#pragma unroll
+        for (int h = 0; h < R / 8; ++h) {
+            const uint4 c4 = cp[h];
+            cw[q][4 * h] = c4.x;
+            cw[q][4 * h + 1] = c4.y;
+            cw[q][4 * h + 2] = c4.z;
+            cw[q][4 * h + 3] = c4.w;
+        }
+    };
+    load_descs(0, cur_d, cur_b);
+#pragma unroll
+    for (int q = 0; q < SPW; ++q)
+        load_entries(q, 0, __builtin_amdgcn_readlane(cur_d.y, q), __builtin_amdgcn_readlane(cur_d.z, q));
+    if (nwin > 1) load_descs(1, nxt_d, nxt_b);
+    for (int64_t k = 0; k < nwin; ++k) {
+        const int32_t kw = (int32_t)k;
+#pragma unroll
+        for (int q = 0; q < SPW; ++q) {
+            const int32_t len = __builtin_amdgcn_readlane(cur_d.z, q) - __builtin_amdgcn_readlane(cur_d.y, q);
+#pragma unroll
+            for (int u = 0; u < R; ++u) xv[q][u] = *(lane + WAVE * u < len ? x + jc[q][u] : &g_win_zero);
+        }
+#pragma unroll
+        for (int q = 0; q < SPW; ++q) {
+            const int i = wv + NW * q;
+            if (k * W + i >= nsteps) break;
+            const int d = __builtin_amdgcn_readlane(cur_d.w, q);
+            const int32_t len = __builtin_amdgcn_readlane(cur_d.z, q) - __builtin_amdgcn_readlane(cur_d.y, q);
+            if (d >= 0 && !bailed) {
+                // one LDS access per poll: lanes 0..d read their step's flag, the others the stop flag
+                int32_t spins = 0;
+                while (true) {
+                    const int32_t v = __hip_atomic_load(&done[lane <= d ? lane : W], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const int32_t f = lane <= d ? v : (v == 0 ? kw : ~kw);
+                    if (__builtin_amdgcn_readlane(f, WAVE - 1) != kw) {   // stopped elsewhere
+                        bailed = true;
+                        break;
+                    }
+                    if (spins >= spin_limit) {
+                        __hip_atomic_store(&done[W], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (lane == 0) atomicExch(err, 1);
+                        bailed = true;
+                        break;
+                    }
+                    if (__all(f == kw)) break;
+                    ++spins;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                if (!bailed) {
+                    // each read lands in its slot's register (no temporary), so all are in flight
+#pragma unroll
+                    for (int u = 0; u < R; ++u) {
+                        const int32_t c = code(q, u);
+                        if (win_haspred(c)) xv[q][u] = pub[c & WIN_SLOT_MASK];
+                    }
+                }
+            }
+            if (!bailed) {
+                __builtin_amdgcn_s_setprio(2);
+                double dot = 0.0;
+#pragma unroll
+                for (int u = 0; u < R; ++u) dot += xv[q][u] * vc[q][u];
+                dot = win_wave_sum(dot);
+                const double coeff = (relax * (lane_value(cur_b.x, q) - dot)) / lane_value(cur_b.y, q);
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    double v = xv[q][u] + coeff * vc[q][u];
+                    if (lwrb) v = fmax_ref(v, lbv);
+                    if (upb) v = fmin_ref(v, ubv);
+                    xv[q][u] = v;
+                }
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    const int32_t c = code(q, u);
+                    if (win_shared(c)) pub[c & WIN_SLOT_MASK] = xv[q][u];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                if (lane == 0) __hip_atomic_store(&done[i], kw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __builtin_amdgcn_s_setprio(0);
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    const int32_t c = code(q, u);
+                    if (lane + WAVE * u < len && (!win_shared(c) || (c & WIN_LAST))) x[jc[q][u]] = xv[q][u];
+                }
+            }
+            if (k + 1 < nwin)
+                load_entries(q, k + 1, __builtin_amdgcn_readlane(nxt_d.y, q), __builtin_amdgcn_readlane(nxt_d.z, q));
+        }
+        __syncthreads();
+        cur_d = nxt_d;
+        cur_b = nxt_b;
+        if (k + 2 < nwin) load_descs(k + 2, nxt_d, nxt_b);
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+}  // namespace
+
+bool art_window_usable(Csr &A, int64_t nsteps) {
+    if (!env_int("FAIR_ART_WINDOW", 1) || A.n <= 0 || nsteps < 2) return false;
+    if (A.n_rows_nz < 0) build_row_list(A);
+    return A.max_row <= 16 * WAVE;
+}
+
+// The window sweep over up to two chains (twin_art's x and y), each with its own sequence.
+void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const int64_t nsteps[2],
+                       double *const x[2], const double *b, const double *nrA, double relax, int lwrb, double lb,
+                       int upb, double ub) {
+    if (A.n_rows_nz < 0) build_row_list(A);
+    const int R = A.max_row <= 8 * WAVE ? 8 : 16;
+    FAIR_CHECK(A.max_row <= 16 * WAVE, FAIR_ERR_ARG, "row too long for the window sweep");
+    ArtWindowBufs &wb = A.win;
+    const int64_t n = A.n;
+    // pre-pass tables: 8 bytes per pixel per workgroup, at most 256 MB
+    const int G = (int)std::max<int64_t>(8, std::min<int64_t>(128, ((int64_t)256 << 20) / (8 * n)));
+    if (wb.G != G || wb.n != n) {
+        wb.mask.alloc((size_t)G * (size_t)n);
+        wb.slotof.alloc((size_t)G * (size_t)n);
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for((int64_t)G * n, 256)), dim3(256), 0, stream(),
+                           wb.mask.p, (int64_t)G * n);
+        launch_check("window tables");
+        wb.G = G;
+        wb.n = n;
+    }
+    if (wb.err.n == 0) wb.err.alloc(1);
+    WinChains ch{};
+    for (int c = 0; c < chains; ++c) {
+        const size_t need = (size_t)nsteps[c] * (size_t)(WAVE * R);
+        if (wb.codes[c].n < need) wb.codes[c].alloc(need);
+        if (wb.desc[c].n < 4 * (size_t)nsteps[c]) wb.desc[c].alloc(4 * (size_t)nsteps[c]);
+        if (wb.bw[c].n < 2 * (size_t)nsteps[c]) wb.bw[c].alloc(2 * (size_t)nsteps[c]);
+        ch.nsteps[c] = nsteps[c];
+        ch.x[c] = x[c];
+        ch.codes[c] = wb.codes[c].p;
+        ch.desc[c] = (const int4 *)wb.desc[c].p;
+        ch.bw[c] = (const double2 *)wb.bw[c].p;
+        const int W = R == 8 ? win_steps<8>() : win_steps<16>();
+        const int64_t nwin = (nsteps[c] + W - 1) / W;
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(G, nwin));
+        if (nsteps[c] <= 0) continue;
+        if (R == 8)
+            hipLaunchKernelGGL(k_win_codes<8>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, stream(), rows[c],
+                               nsteps[c], A.IA.p, A.JA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p, wb.codes[c].p,
+                               (int4 *)wb.desc[c].p, (double2 *)wb.bw[c].p);
+        else
+            hipLaunchKernelGGL(k_win_codes<16>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, stream(), rows[c],
+                               nsteps[c], A.IA.p, A.JA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p, wb.codes[c].p,
+                               (int4 *)wb.desc[c].p, (double2 *)wb.bw[c].p);
+        launch_check("window codes");
+    }
+    for (int c = chains; c < 2; ++c) {
+        ch.nsteps[c] = 0;
+        ch.x[c] = nullptr;
+        ch.codes[c] = nullptr;
+        ch.desc[c] = nullptr;
+        ch.bw[c] = nullptr;
+    }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream(), (uint32_t *)wb.err.p, (int64_t)1);
+    const int32_t spin = std::max(0, env_int("FAIR_ART_WINDOW_SPIN", 1 << 22));
+    // the first spare entry after the matrix's entries (JA = 0, vA = 0, see alloc_entries)
+    const int32_t pad = (int32_t)A.nnz;
+    if (R == 8)
+        hipLaunchKernelGGL(k_art_window<8>, dim3(chains), dim3(WinShape<8>::NW * WAVE), 0, stream(), ch, A.JA.p,
+                           A.vA.p, pad, relax, lwrb, lb, upb, ub, spin, wb.err.p);
+    else
+        hipLaunchKernelGGL(k_art_window<16>, dim3(chains), dim3(WinShape<16>::NW * WAVE), 0, stream(), ch, A.JA.p,
+                           A.vA.p, pad, relax, lwrb, lb, upb, ub, spin, wb.err.p);
+    launch_check("window sweep");
+    if (env_int("FAIR_DEBUG", 0))
+        fprintf(stderr, "[fair] window sweep: %lld + %lld steps, windows of %d, %d entries per lane\n",
+                (long long)nsteps[0], (long long)(chains > 1 ? nsteps[1] : 0),
+                R == 8 ? win_steps<8>() : win_steps<16>(), R);
+    int32_t herr = 0;
+    FAIR_HIP(hipMemcpyAsync(&herr, wb.err.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
+    FAIR_HIP(hipStreamSynchronize(stream()));
+    FAIR_CHECK(herr == 0, FAIR_ERR_HIP, "Kaczmarz window sweep: a step waited beyond its bound");
+}
+
+}  // namespace fair

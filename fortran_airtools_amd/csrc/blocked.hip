@@ -115,11 +115,6 @@ __global__ void __launch_bounds__(256) k_footprint(const int4 *__restrict__ binf
     }
 }
 
-int env_int(const char *name, int dflt) {
-    const char *v = getenv(name);
-    return v && *v ? atoi(v) : dflt;
-}
-
 // Group the non-empty rows by key (stable), split groups into blocks of at most rb rows and
 // build the footprints.
 void build_blocked(Csr &A, const std::vector<int64_t> &key_of_row, int32_t rb, int32_t W, int32_t NB,

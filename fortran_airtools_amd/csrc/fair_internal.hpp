@@ -175,6 +175,19 @@ struct LevelSched {
     }
 };
 
+// Buffers of the Kaczmarz window sweep (artwin.hip): per-step entry codes and descriptors of up
+// to two chains, the pre-pass tables (G workgroups x n pixels each) and the error flag.
+struct ArtWindowBufs {
+    DBuf<uint16_t> codes[2];
+    DBuf<int32_t> desc[2];     // int4 per step
+    DBuf<double> bw[2];        // double2 per step
+    DBuf<uint32_t> mask;
+    DBuf<int32_t> slotof;
+    DBuf<int32_t> err;
+    int G = 0;
+    int64_t n = 0;
+};
+
 struct Csr {
     int32_t m = 0, n = 0;
     int64_t nnz = 0;
@@ -203,6 +216,7 @@ struct Csr {
     std::vector<std::shared_ptr<LevelSched>> lev;
     HostArray<int32_t> hIA, hJA;                    // host copy of the pattern for level planning
                                                     // (kept up to 64 M entries)
+    ArtWindowBufs win;
 };
 
 }  // namespace fair
@@ -262,6 +276,19 @@ inline void check_unique_columns(const fair_csr *h) {
 }
 void ensure_T(fair_csr *h);
 void invalidate_T(fair_csr *h);
+
+// Integer from the environment (dflt when unset or empty).
+inline int env_int(const char *name, int dflt) {
+    const char *v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
+// Kaczmarz window sweep (artwin.hip): the row steps of up to two chains (one-based rows on the
+// device), each iterate updated in place, bit-identical to the one-wavefront sweep.  Synchronous.
+bool art_window_usable(Csr &A, int64_t nsteps);   // FAIR_ART_WINDOW, rows up to 1024 entries
+void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const int64_t nsteps[2],
+                       double *const x[2], const double *b, const double *nrA, double relax, int lwrb, double lb,
+                       int upb, double ub);
 
 // y[r] = sum_k vA[k] * x[JA[k]] over every row; empty rows give 0.
 void spmv(const Csr &A, const double *x, double *y);

@@ -387,6 +387,22 @@ void launch_art_sweep(Csr &A, unsigned chains, Args... args) {
         hipLaunchKernelGGL(k_art_sweep<8>, dim3(chains), dim3(SWEEP_THREADS), 0, stream(), args...);
 }
 
+// Row steps of up to two chains without a level schedule: the window sweep (artwin.hip) where it
+// applies, else the one-wavefront sweep; both give the same iterate bit for bit.
+void launch_art_chains(Csr &A, int chains, const int32_t *r0, int64_t n0, double *x0, const int32_t *r1, int64_t n1,
+                       double *x1, const double *b, const double *nrA, double relax, const Box &box) {
+    if (art_window_usable(A, std::max(n0, n1))) {
+        const int32_t *const rows[2] = {r0, r1};
+        const int64_t ns[2] = {n0, n1};
+        double *const xs[2] = {x0, x1};
+        launch_art_window(A, chains, rows, ns, xs, b, nrA, relax, box.lwrb, box.lb, box.upb, box.ub);
+        return;
+    }
+    launch_art_sweep(A, (unsigned)chains, SweepChains{{r0, r1}, {n0, n1}, {x0, x1}}, A.IA.p, A.JA.p, A.vA.p, b, nrA,
+                     A.n, relax, box.lwrb, box.lb, box.upb, box.ub);
+    launch_check("art sweep");
+}
+
 // ---- level-parallel sweep ----------------------------------------------------------------
 // The same row steps as k_art_sweep, in an order that keeps every pixel's update sequence: the
 // steps of one level (LevelSched) touch disjoint pixels.  Two executors: one launch per level
@@ -1262,10 +1278,7 @@ extern "C" int fair_art_steps(fair_csr *h, const int32_t *rows, int64_t nsteps, 
         if (lev) {
             launch_art_levels(h->A, lev.get(), dx.p, nullptr, nullptr, db.p, dw.p, relax, box);
         } else {
-            launch_art_sweep(h->A, 1,
-                               SweepChains{{dr.p, nullptr}, {nsteps, 0}, {dx.p, nullptr}}, h->A.IA.p, h->A.JA.p,
-                               h->A.vA.p, db.p, dw.p, n, relax, box.lwrb, box.lb, box.upb, box.ub);
-            launch_check("art sweep");
+            launch_art_chains(h->A, 1, dr.p, nsteps, dx.p, nullptr, 0, nullptr, db.p, dw.p, relax, box);
         }
         dx.download(x, n);
         FAIR_HIP(hipStreamSynchronize(stream()));
@@ -1282,10 +1295,7 @@ extern "C" int fair_art_steps_dev(fair_csr *h, const int32_t *d_rows, int64_t ns
         const Box box = make_box(o);
         const double relax = o ? o->relaxpar : 1.0;
         if (nsteps > 0 && h->A.n > 0)
-            launch_art_sweep(h->A, 1,
-                               SweepChains{{d_rows, nullptr}, {nsteps, 0}, {d_x, nullptr}}, h->A.IA.p, h->A.JA.p,
-                               h->A.vA.p, d_b, d_nrA, h->A.n, relax, box.lwrb, box.lb, box.upb, box.ub);
-        launch_check("art sweep");
+            launch_art_chains(h->A, 1, d_rows, nsteps, d_x, nullptr, 0, nullptr, d_b, d_nrA, relax, box);
     });
 }
 
@@ -1372,11 +1382,7 @@ extern "C" int fair_twin_art(fair_csr *h, int32_t method, int32_t stoprule, cons
             }
             dsx.upload(sx.data(), per_chain);
             dsy.upload(sy.data(), per_chain);
-            launch_art_sweep(h->A, 2,
-                               SweepChains{{dsx.p, dsy.p}, {(int64_t)per_chain, (int64_t)per_chain}, {x.p, y.p}},
-                               h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p, n, relax, box.lwrb, box.lb, box.upb,
-                               box.ub);
-            launch_check("twin sweep");
+            launch_art_chains(h->A, 2, dsx.p, per_chain, x.p, dsy.p, per_chain, y.p, db.p, d_nrA.p, relax, box);
         };
         auto norm_diff = [&](const double *a, const double *c) { return std::sqrt(dev_sum(n, SqDiff{a, c}, part, out)); };
         if (P.isrand) scramble_now();                                   // :620-622
@@ -1538,11 +1544,8 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
             launch_art_levels(h->A, lev.get(), dx.p, nullptr, nullptr, db.p, d_nrA.p, relax, box);
         } else if (steps_per_iter > 0) {
             dseq.upload(seq.data(), steps_per_iter);
-            launch_art_sweep(h->A, 1,
-                               SweepChains{{dseq.p, nullptr}, {(int64_t)steps_per_iter, 0}, {dx.p, nullptr}},
-                               h->A.IA.p, h->A.JA.p, h->A.vA.p, db.p, d_nrA.p, h->A.n, relax, box.lwrb, box.lb,
-                               box.upb, box.ub);
-            launch_check("art sweep");
+            launch_art_chains(h->A, 1, dseq.p, steps_per_iter, dx.p, nullptr, 0, nullptr, db.p, d_nrA.p, relax,
+                              box);
         }
         if (xref) {                                   // :330-333
             const double cur = std::sqrt(dev_sum(n, SqDiff{dx.p, dxr.p}, part, out));

@@ -50,7 +50,8 @@ def rhs(A, seed):
 
 
 def both_paths(monkeypatch, capfd, fn, width="1"):
-    """fn() with the level sweep (FAIR_ART_LEVEL_WIDTH=width) and with the one-wavefront sweep;
+    """fn() with the level sweep (FAIR_ART_LEVEL_WIDTH=width) and with the one-wavefront sweep
+    (FAIR_ART_LEVELS=0, FAIR_ART_WINDOW=0);
     returns both results and whether the level path printed its line."""
     monkeypatch.setenv("FAIR_DEBUG", "1")
     monkeypatch.setenv("FAIR_ART_LEVEL_WIDTH", width)
@@ -58,8 +59,10 @@ def both_paths(monkeypatch, capfd, fn, width="1"):
     lev = fn()
     used = "level sweep" in capfd.readouterr().err
     monkeypatch.setenv("FAIR_ART_LEVELS", "0")
+    monkeypatch.setenv("FAIR_ART_WINDOW", "0")
     seq = fn()
     monkeypatch.delenv("FAIR_ART_LEVELS")
+    monkeypatch.delenv("FAIR_ART_WINDOW")
     return lev, seq, used
 
 
