@@ -39,7 +39,7 @@ constexpr int WIN_PRE_WAVES = 8;   // pre-pass workgroup: 512 threads
 // Rows up to 64 R entries.  LDS holds a double per shared pixel: at most half of a window's
 // entries, W * 64 R / 2 (64 KB for both shapes).
 template <int R> struct WinShape;
-template <> struct WinShape<8> { static constexpr int NW = 8, SPW = 3; };
+template <> struct WinShape<8> { static constexpr int NW = 8, SPW = 4; };
 template <> struct WinShape<16> { static constexpr int NW = 8, SPW = 1; };
 
 template <int R> constexpr int win_steps() { return WinShape<R>::NW * WinShape<R>::SPW; }
@@ -50,6 +50,8 @@ struct WinChains {
     double *x[2];
     const int4 *desc[2];        // per step {row (zero-based), IA(row), IA(row+1), dep}
     const double2 *bw[2];       // per step {b(row), nrA(row)}
+    const int32_t *cslot[2];    // per step: LDS slots of its first 64 shared entries
+    const double *cval[2];      // per step: their matrix values
     const uint16_t *codes[2];   // 64 R per step, lane-major (the R codes of a lane are adjacent)
 };
 
@@ -63,9 +65,10 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
 // clears the entries it set).  Within a window:
 //   A  every entry ORs its step's bit into mask[pixel];
 //   B  every entry reads the mask; the first step of a shared pixel takes an LDS slot number
-//      and publishes it in slotof[pixel]; each step's dep is the largest earlier bit, written
-//      with the step's row, entry range, b and nrA;
-//   C  every entry writes its code;
+//      and publishes it in slotof[pixel]; each step's dep is the largest earlier bit;
+//   C  every entry writes its code; each step lists its shared entries (LDS slot and matrix
+//      value, the first 64 in slot order) and writes its descriptor {number of shared entries,
+//      entry range, dep}, b and nrA;
 //   D  every entry clears mask[pixel].
 // The tables are accessed with device-scope atomics, and a fence and barrier separate the
 // phases.
@@ -74,11 +77,13 @@ __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_
                                                                     int64_t nsteps,
                                                                     const int32_t *__restrict__ IA,
                                                                     const int32_t *__restrict__ JA,
+                                                                    const double *__restrict__ vA,
                                                                     const double *__restrict__ b,
                                                                     const double *__restrict__ nrA,
                                                                     int32_t ncols, uint32_t *mask_all,
                                                                     int32_t *slot_all, uint16_t *codes,
-                                                                    int4 *desc, double2 *bw) {
+                                                                    int4 *desc, double2 *bw, int32_t *cslot,
+                                                                    double *cval) {
     constexpr int W = win_steps<R>();
     constexpr int SPW = W / WIN_PRE_WAVES;
     static_assert(W % WIN_PRE_WAVES == 0 && W <= 32, "window shape");
@@ -93,7 +98,7 @@ __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_
     for (int64_t k = blockIdx.x; k < nwin; k += gridDim.x) {
         int32_t jc[SPW][R];
         uint32_t mk[SPW][R];
-        int32_t rq[SPW], bq[SPW], eq[SPW];
+        int32_t rq[SPW], bq[SPW], eq[SPW], dq[SPW];
 #pragma unroll
         for (int q = 0; q < SPW; ++q) {
             const int i = wv + WIN_PRE_WAVES * q;
@@ -136,11 +141,7 @@ __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_
             }
 #pragma unroll
             for (int off = WAVE / 2; off > 0; off >>= 1) d = max(d, __shfl_xor(d, off, WAVE));
-            const int64_t s = k * W + i;
-            if (lane == 0 && s < nsteps) {
-                desc[s] = make_int4(rq[q], bq[q], eq[q], d);
-                bw[s] = make_double2(b[rq[q]], nrA[rq[q]]);
-            }
+            dq[q] = d;
         }
         __threadfence();
         __syncthreads();
@@ -149,17 +150,33 @@ __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_
             const int i = wv + WIN_PRE_WAVES * q;
             const int64_t s = k * W + i;
             if (s >= nsteps) continue;
+            int32_t nsh = 0;   // shared entries listed so far (wavefront-uniform)
 #pragma unroll
             for (int u = 0; u < R; ++u) {
                 const int32_t j = jc[q][u];
                 const uint32_t M = mk[q][u];
                 uint16_t c = WIN_UNSHARED;
-                if (__popc(M) >= 2) {
-                    const int32_t sl = __hip_atomic_load(&slotof[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int32_t sl = 0;
+                const bool sh = __popc(M) >= 2;
+                if (sh) {
+                    sl = __hip_atomic_load(&slotof[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     c = (uint16_t)((sl & WIN_SLOT_MASK) | ((M & ((1u << i) - 1u)) ? WIN_HASPRED : 0) |
                                    ((M >> i) == 1u ? WIN_LAST : 0));
                 }
                 codes[s * (WAVE * R) + lane * R + u] = c;
+                const uint64_t bal = __ballot(sh);
+                const int32_t pos =
+                    nsh + (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                if (sh && pos < WAVE) {
+                    cslot[s * WAVE + pos] = sl & WIN_SLOT_MASK;
+                    cval[s * WAVE + pos] = vA[bq[q] + lane + WAVE * u];
+                }
+                nsh += __popcll(bal);
+            }
+            if (lane == 0) {
+                desc[s] = make_int4(nsh, bq[q], eq[q], dq[q]);
+                bw[s] = make_double2(b[rq[q]], nrA[rq[q]]);
             }
         }
         __syncthreads();   // every slot number is read; nslot is no longer used by this window
@@ -229,13 +246,18 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
     const int4 *__restrict__ desc = ch.desc[blockIdx.x];
     const double2 *__restrict__ bw = ch.bw[blockIdx.x];
     const uint16_t *__restrict__ codes = ch.codes[blockIdx.x];
+    const int32_t *__restrict__ cslot = ch.cslot[blockIdx.x];
+    const double *__restrict__ cval = ch.cval[blockIdx.x];
     if (threadIdx.x <= W) done[threadIdx.x] = threadIdx.x < W ? -1 : 0;
     __syncthreads();
     const int64_t nwin = (nsteps + W - 1) / W;
     if (nwin == 0) return;   // the whole workgroup leaves together
     int32_t jc[SPW][R];
     uint32_t cw[SPW][R / 2];   // the lane's codes, two per word
-    double vc[SPW][R], xv[SPW][R];
+    double vc[SPW][R];
+    double xv[R];              // x at the pixels of the step whose turn is next
+    int32_t ccs[SPW];          // lane t: LDS slot and matrix value of the step's shared entry t
+    double cva[SPW];
     // Descriptors of the wavefront's steps, lane q holding step q: {beg, end, dep} and {b, nrA} of
     // this window (cur) and of the next (nxt), loaded a window ahead by one vector load each.
     int4 cur_d, nxt_d;
@@ -268,6 +290,8 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
             jc[q][u] = JA[ee];
             vc[q][u] = vA[ee];
         }
+        ccs[q] = cslot[sc * WAVE + lane];
+        cva[q] = cval[sc * WAVE + lane];
         const uint4 *cp = reinterpret_cast<const uint4 *>(codes + sc * (WAVE * R) + lane * R);
 #pragma unroll
         for (int h = 0; h < R / 8; ++h) {
@@ -285,17 +309,30 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
     if (nwin > 1) load_descs(1, nxt_d, nxt_b);
     for (int64_t k = 0; k < nwin; ++k) {
         const int32_t kw = (int32_t)k;
-#pragma unroll
-        for (int q = 0; q < SPW; ++q) {
+        // x of a step's pixels is loaded just before the wavefront's turn before it: no store of
+        // this window can reach them first (an unshared pixel has no other writer; a shared one is
+        // stored by its last occurrence, which waits for this step, or read from LDS)
+        auto gather = [&](int q) {
             const int32_t len = __builtin_amdgcn_readlane(cur_d.z, q) - __builtin_amdgcn_readlane(cur_d.y, q);
 #pragma unroll
-            for (int u = 0; u < R; ++u) xv[q][u] = *(lane + WAVE * u < len ? x + jc[q][u] : &g_win_zero);
-        }
+            for (int u = 0; u < R; ++u) xv[u] = *(lane + WAVE * u < len ? x + jc[q][u] : &g_win_zero);
+        };
+        gather(0);
 #pragma unroll
         for (int q = 0; q < SPW; ++q) {
             const int i = wv + NW * q;
             if (k * W + i >= nsteps) break;
             const int d = __builtin_amdgcn_readlane(cur_d.w, q);
+            const int32_t nsh = __builtin_amdgcn_readlane(cur_d.x, q);
+            const bool compact = nsh <= WAVE;
+            // a first occurrence of a shared pixel puts the window-start value in the pixel's LDS
+            // slot (a later occurrence reads it there once this step is done)
+            if (!bailed)
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    const int32_t c = code(q, u);
+                    if (win_shared(c) && !(c & WIN_HASPRED)) pub[c & WIN_SLOT_MASK] = xv[u];
+                }
             const int32_t len = __builtin_amdgcn_readlane(cur_d.z, q) - __builtin_amdgcn_readlane(cur_d.y, q);
             if (d >= 0 && !bailed) {
                 // one LDS access per poll: lanes 0..d read their step's flag, the others the stop flag
@@ -323,38 +360,63 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
 #pragma unroll
                     for (int u = 0; u < R; ++u) {
                         const int32_t c = code(q, u);
-                        if (win_haspred(c)) xv[q][u] = pub[c & WIN_SLOT_MASK];
+                        if (win_haspred(c)) xv[u] = pub[c & WIN_SLOT_MASK];
                     }
                 }
             }
+            // the current values of the listed shared entries (lane t: entry t)
+            const double cv = pub[lane < nsh ? ccs[q] : 0];
             if (!bailed) {
                 __builtin_amdgcn_s_setprio(2);
                 double dot = 0.0;
 #pragma unroll
-                for (int u = 0; u < R; ++u) dot += xv[q][u] * vc[q][u];
+                for (int u = 0; u < R; ++u) dot += xv[u] * vc[q][u];
                 dot = win_wave_sum(dot);
                 const double coeff = (relax * (lane_value(cur_b.x, q) - dot)) / lane_value(cur_b.y, q);
+                auto update_all = [&] {
 #pragma unroll
-                for (int u = 0; u < R; ++u) {
-                    double v = xv[q][u] + coeff * vc[q][u];
-                    if (lwrb) v = fmax_ref(v, lbv);
-                    if (upb) v = fmin_ref(v, ubv);
-                    xv[q][u] = v;
-                }
+                    for (int u = 0; u < R; ++u) {
+                        double v = xv[u] + coeff * vc[q][u];
+                        if (lwrb) v = fmax_ref(v, lbv);
+                        if (upb) v = fmin_ref(v, ubv);
+                        xv[u] = v;
+                    }
+                };
+                if (compact) {
+                    // only the shared pixels are needed before the step is marked done
+                    if (lane < nsh) {
+                        double v = cv + coeff * cva[q];
+                        if (lwrb) v = fmax_ref(v, lbv);
+                        if (upb) v = fmin_ref(v, ubv);
+                        pub[ccs[q]] = v;
+                    }
+                } else {
+                    update_all();
 #pragma unroll
-                for (int u = 0; u < R; ++u) {
-                    const int32_t c = code(q, u);
-                    if (win_shared(c)) pub[c & WIN_SLOT_MASK] = xv[q][u];
+                    for (int u = 0; u < R; ++u) {
+                        const int32_t c = code(q, u);
+                        if (win_shared(c)) pub[c & WIN_SLOT_MASK] = xv[u];
+                    }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
                 if (lane == 0) __hip_atomic_store(&done[i], kw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 __builtin_amdgcn_s_setprio(0);
+                // slots past the row (u >= nsl) hold no entry in any lane: skipped by a uniform test
+                const int nsl = (len + WAVE - 1) / WAVE;
 #pragma unroll
                 for (int u = 0; u < R; ++u) {
+                    if (u >= nsl) break;
+                    double v = xv[u];
+                    if (compact) {
+                        v = v + coeff * vc[q][u];
+                        if (lwrb) v = fmax_ref(v, lbv);
+                        if (upb) v = fmin_ref(v, ubv);
+                    }
                     const int32_t c = code(q, u);
-                    if (lane + WAVE * u < len && (!win_shared(c) || (c & WIN_LAST))) x[jc[q][u]] = xv[q][u];
+                    if (lane + WAVE * u < len && (!win_shared(c) || (c & WIN_LAST))) x[jc[q][u]] = v;
                 }
             }
+            if (q + 1 < SPW && k * W + i + NW < nsteps) gather(q + 1);
             if (k + 1 < nwin)
                 load_entries(q, k + 1, __builtin_amdgcn_readlane(nxt_d.y, q), __builtin_amdgcn_readlane(nxt_d.z, q));
         }
@@ -405,23 +467,29 @@ void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const i
         if (wb.codes[c].n < need) wb.codes[c].alloc(need);
         if (wb.desc[c].n < 4 * (size_t)nsteps[c]) wb.desc[c].alloc(4 * (size_t)nsteps[c]);
         if (wb.bw[c].n < 2 * (size_t)nsteps[c]) wb.bw[c].alloc(2 * (size_t)nsteps[c]);
+        if (wb.cslot[c].n < (size_t)nsteps[c] * WAVE) wb.cslot[c].alloc((size_t)nsteps[c] * WAVE);
+        if (wb.cval[c].n < (size_t)nsteps[c] * WAVE) wb.cval[c].alloc((size_t)nsteps[c] * WAVE);
         ch.nsteps[c] = nsteps[c];
         ch.x[c] = x[c];
         ch.codes[c] = wb.codes[c].p;
         ch.desc[c] = (const int4 *)wb.desc[c].p;
         ch.bw[c] = (const double2 *)wb.bw[c].p;
+        ch.cslot[c] = wb.cslot[c].p;
+        ch.cval[c] = wb.cval[c].p;
         const int W = R == 8 ? win_steps<8>() : win_steps<16>();
         const int64_t nwin = (nsteps[c] + W - 1) / W;
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(G, nwin));
         if (nsteps[c] <= 0) continue;
         if (R == 8)
             hipLaunchKernelGGL(k_win_codes<8>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, stream(), rows[c],
-                               nsteps[c], A.IA.p, A.JA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p, wb.codes[c].p,
-                               (int4 *)wb.desc[c].p, (double2 *)wb.bw[c].p);
+                               nsteps[c], A.IA.p, A.JA.p, A.vA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p,
+                               wb.codes[c].p, (int4 *)wb.desc[c].p, (double2 *)wb.bw[c].p, wb.cslot[c].p,
+                               wb.cval[c].p);
         else
             hipLaunchKernelGGL(k_win_codes<16>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, stream(), rows[c],
-                               nsteps[c], A.IA.p, A.JA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p, wb.codes[c].p,
-                               (int4 *)wb.desc[c].p, (double2 *)wb.bw[c].p);
+                               nsteps[c], A.IA.p, A.JA.p, A.vA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p,
+                               wb.codes[c].p, (int4 *)wb.desc[c].p, (double2 *)wb.bw[c].p, wb.cslot[c].p,
+                               wb.cval[c].p);
         launch_check("window codes");
     }
     for (int c = chains; c < 2; ++c) {
@@ -430,6 +498,8 @@ void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const i
         ch.codes[c] = nullptr;
         ch.desc[c] = nullptr;
         ch.bw[c] = nullptr;
+        ch.cslot[c] = nullptr;
+        ch.cval[c] = nullptr;
     }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream(), (uint32_t *)wb.err.p, (int64_t)1);
     const int32_t spin = std::max(0, env_int("FAIR_ART_WINDOW_SPIN", 1 << 22));

@@ -181,6 +181,8 @@ struct ArtWindowBufs {
     DBuf<uint16_t> codes[2];
     DBuf<int32_t> desc[2];     // int4 per step
     DBuf<double> bw[2];        // double2 per step
+    DBuf<int32_t> cslot[2];    // 64 per step: the step's shared entries (LDS slot, value)
+    DBuf<double> cval[2];
     DBuf<uint32_t> mask;
     DBuf<int32_t> slotof;
     DBuf<int32_t> err;
