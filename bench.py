@@ -94,7 +94,8 @@ def run_cpu_baseline(geom=None, legs=("sart@16",), time_build=False, iters: int 
 
     env = dict(os.environ, OMP_STACKSIZE="256M")
     try:
-        # its progress lines pass through on stderr; the result is the last stdout line
+        # its progress lines pass through on stderr; the result is the last stdout line that holds
+        # a JSON object (the reference's own output, flushed at exit, may follow it)
         r = subprocess.run(cmd, cwd=ROOT, stdout=subprocess.PIPE, text=True, timeout=600,
                            preexec_fn=big_stack, env=env)
     except subprocess.TimeoutExpired:
@@ -103,7 +104,11 @@ def run_cpu_baseline(geom=None, legs=("sart@16",), time_build=False, iters: int 
     if r.returncode != 0:
         log(f"cpu baseline failed (status {r.returncode})")
         return None
-    return json.loads(r.stdout.strip().splitlines()[-1])
+    for line in reversed(r.stdout.strip().splitlines()):
+        if line.startswith("{"):
+            return json.loads(line)
+    log("cpu baseline printed no result")
+    return None
 
 
 def leg(d: dict | None, i: int) -> dict | None:
@@ -708,6 +713,24 @@ def bench_kaczmarz(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
                  "first_call_s": round(t_first, 4), "first_call_sweeps": 1, "relerr_vs_oracle": err,
                  "within_1e-12": err <= 1e-12, "reference_host_s": None if t_ref is None else round(t_ref, 4),
                  "reference_sample": None if ref2 is None else ref2["sample"]})
+    # uniform randomized Kaczmarz on the same problem (rows up to about 1000 entries: the window
+    # sweep with 16 entries per lane), 2 sweeps; the reference's own art on one host thread
+    u2 = np.random.default_rng(6).random(airms.art_uniform_count("randkaczmarz", A2, [2], options=o))
+    airms.art("randkaczmarz", A2, b2, [1], options=o, uniforms=u2)                 # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    X3 = airms.art("randkaczmarz", A2, b2, [2], options=o, uniforms=u2)
+    t_gpu = time.perf_counter() - t0
+    Xo3, _ = orc.art("randkaczmarz", I2, J2, v2, A2.n, b2, [2], u=u2, lb=0.0, ub=1.0)
+    err = float(np.linalg.norm(X3[0] - Xo3[0]) / np.linalg.norm(Xo3[0]))
+    ref3 = None if a.no_cpu else cpu_baseline(1, "randkaczmarz", geom=g2)
+    t_ref = None if ref3 is None else 2.0 / ref3["value"]
+    log(f"randkaczmarz N=512: GPU {t_gpu:.3f} s for 2 sweeps, rel. error to oracle {err:.2e}"
+        + (f"; reference on one host thread {t_ref:.3f} s" if t_ref else ""))
+    rows.append({"method": "randkaczmarz", "n": g2.n, "sweeps": 2, "gpu_s": round(t_gpu, 4),
+                 "relerr_vs_oracle": err, "within_1e-12": err <= 1e-12,
+                 "reference_host_s": None if t_ref is None else round(t_ref, 4),
+                 "reference_sample": None if ref3 is None else ref3["sample"]})
     out = {"metric": METRIC, "mode": "kaczmarz", "unit": "rows/s", "dtype": "f64",
            "config": {"workload": "N=256 fan beam 360 x 363 rays, Kaczmarz 20 sweeps, box [0,1]",
                       "nnz": A.nnz, "nonempty_rows": nonempty},

@@ -6,8 +6,8 @@
 // stream per step.  Here the work of a window that does not depend on the order is done side by
 // side: every step's entries and x values are loaded at the window's start, and only the values
 // of pixels that two or more steps of the window touch ("shared" pixels) pass from step to step,
-// through LDS.  A step waits only until every earlier step of its window up to the last one that
-// shares a pixel with it is done, so steps that do not conflict run at the same time.
+// through LDS.  A step waits only for the earlier steps of its window that share a pixel with it, so
+// steps that do not conflict run at the same time.
 //
 // A pre-pass kernel classifies each entry of each step (per window, from a bit mask of the steps
 // touching each pixel):
@@ -16,7 +16,7 @@
 //   slot | P << 14 | L << 15   shared: P = an earlier step of the window touched it (read the
 //                              value from LDS slot `slot`, else use the window-start value),
 //                              L = no later step touches it (store the result to x);
-// and records per step dep = the latest earlier step of its window sharing a pixel (-1: none).
+// and records per step the mask of the earlier steps of its window that share a pixel with it.
 // The executor then computes every row exactly as k_art_sweep does (entries of lane q:
 // beg + q + 64 t in increasing t, products and sums rounded separately, the DPP wavefront sum,
 // coeff = relax*(b - dot)/nrA, x + coeff*a then max(lb), min(ub)), on the same values in the
@@ -65,13 +65,16 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
 // clears the entries it set).  Within a window:
 //   A  every entry ORs its step's bit into mask[pixel];
 //   B  every entry reads the mask; the first step of a shared pixel takes an LDS slot number
-//      and publishes it in slotof[pixel]; each step's dep is the largest earlier bit;
+//      and publishes it in slotof[pixel]; each step's dependency mask is the OR of the earlier
+//      bits of its pixels;
 //   C  every entry writes its code; each step lists its shared entries (LDS slot and matrix
 //      value, the first 64 in slot order) and writes its descriptor {number of shared entries,
-//      entry range, dep}, b and nrA;
+//      entry range, dependency mask}, b and nrA;
 //   D  every entry clears mask[pixel].
-// The tables are accessed with device-scope atomics, and a fence and barrier separate the
-// phases.
+// The tables are private to the workgroup and every access to them is performed in the L2 (atomics,
+// and loads and stores at device scope, which bypass the CU's L1): each wavefront waits until its
+// own accesses are done before the barrier that ends a phase, and no cache write-back is needed
+// (a device-wide fence would write back the L2 in every phase, slowing a sweep running beside).
 template <int R>
 __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_t *__restrict__ rows,
                                                                     int64_t nsteps,
@@ -119,12 +122,12 @@ __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_
                                                 __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        __threadfence();
+        __builtin_amdgcn_s_waitcnt(0);   // this wavefront's table accesses are done
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < SPW; ++q) {
             const int i = wv + WIN_PRE_WAVES * q;
-            int d = -1;
+            uint32_t dm = 0;
 #pragma unroll
             for (int u = 0; u < R; ++u) {
                 const int32_t j = jc[q][u];
@@ -136,14 +139,14 @@ __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_
                         __hip_atomic_store(&slotof[j], sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                     const uint32_t below = M & ((1u << i) - 1u);
-                    if (below) d = max(d, 31 - __clz(below));
+                    dm |= below;
                 }
             }
 #pragma unroll
-            for (int off = WAVE / 2; off > 0; off >>= 1) d = max(d, __shfl_xor(d, off, WAVE));
-            dq[q] = d;
+            for (int off = WAVE / 2; off > 0; off >>= 1) dm |= (uint32_t)__shfl_xor((int)dm, off, WAVE);
+            dq[q] = (int32_t)dm;
         }
-        __threadfence();
+        __builtin_amdgcn_s_waitcnt(0);   // this wavefront's table accesses are done
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < SPW; ++q) {
@@ -187,13 +190,15 @@ __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_
                 if (jc[q][u] >= 0)
                     __hip_atomic_store(&mask[jc[q][u]], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (threadIdx.x == 0) nslot = 0;
-        __threadfence();
+        __builtin_amdgcn_s_waitcnt(0);   // this wavefront's table accesses are done
         __syncthreads();
     }
 }
 
-// wave_sum_dpp's sum (the same lane pairs in the same order, so the same bits) with DPP moves
-// that need no zero-filled destination.
+// wave_sum_dpp's sum with DPP moves that need no zero-filled destination, the four row sums
+// joined in the vector unit: row_bcast:15 adds row 0 to row 1 and row 2 to row 3, row_bcast:31
+// then adds row 1 to row 3, so lane 63 holds (r3 + r2) + (r1 + r0), which is wave_sum_dpp's
+// (r0 + r1) + (r2 + r3) bit for bit (each add's operands are the same two values).
 template <int CTRL>
 __device__ __forceinline__ double win_dpp(double v) {
     const long long b = __double_as_longlong(v);
@@ -206,7 +211,9 @@ __device__ __forceinline__ double win_wave_sum(double v) {
     v += win_dpp<0x4E>(v);
     v += win_dpp<0x141>(v);
     v += win_dpp<0x140>(v);
-    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+    v += win_dpp<0x142>(v);   // row_bcast:15 (row 0 reads out of range: 0)
+    v += win_dpp<0x143>(v);   // row_bcast:31 (rows 0 and 1 read out of range: 0)
+    return lane_value(v, 63);
 }
 
 // The value a padded slot gathers (its product must be +0 whatever x holds).
@@ -255,7 +262,10 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
     int32_t jc[SPW][R];
     uint32_t cw[SPW][R / 2];   // the lane's codes, two per word
     double vc[SPW][R];
-    double xv[R];              // x at the pixels of the step whose turn is next
+    // x at the pixels of the next step, or of the next two (by parity of q) where the registers
+    // allow (three steps or fewer per wavefront)
+    constexpr bool TWO = SPW <= 3;
+    double xva[R], xvb[R];
     int32_t ccs[SPW];          // lane t: LDS slot and matrix value of the step's shared entry t
     double cva[SPW];
     // Descriptors of the wavefront's steps, lane q holding step q: {beg, end, dep} and {b, nrA} of
@@ -273,7 +283,7 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
         const int64_t s = k * W + wv + NW * (lane < SPW ? lane : 0);
         const bool ok = s < nsteps;
         const int4 t = desc[ok ? s : 0];
-        d = ok ? t : make_int4(t.x, t.y, t.y, -1);
+        d = ok ? t : make_int4(t.x, t.y, t.y, 0);
         bv = bw[ok ? s : 0];
     };
     // entries and codes of step q of window k, entry range [beg, end).  Slots past the row load
@@ -309,20 +319,23 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
     if (nwin > 1) load_descs(1, nxt_d, nxt_b);
     for (int64_t k = 0; k < nwin; ++k) {
         const int32_t kw = (int32_t)k;
-        // x of a step's pixels is loaded just before the wavefront's turn before it: no store of
-        // this window can reach them first (an unshared pixel has no other writer; a shared one is
-        // stored by its last occurrence, which waits for this step, or read from LDS)
-        auto gather = [&](int q) {
+        // x of a step's pixels is loaded one or two turns of its wavefront ahead (the first at the
+        // window's start): no store of this window can reach them first (an unshared pixel has no
+        // other writer; a shared one is stored by its last occurrence, which waits for this step,
+        // or read from LDS)
+        auto gather = [&](int q, double (&dst)[R]) {
             const int32_t len = __builtin_amdgcn_readlane(cur_d.z, q) - __builtin_amdgcn_readlane(cur_d.y, q);
 #pragma unroll
-            for (int u = 0; u < R; ++u) xv[u] = *(lane + WAVE * u < len ? x + jc[q][u] : &g_win_zero);
+            for (int u = 0; u < R; ++u) dst[u] = *(lane + WAVE * u < len ? x + jc[q][u] : &g_win_zero);
         };
-        gather(0);
+        gather(0, xva);
+        if (TWO && SPW > 1) gather(1, xvb);
 #pragma unroll
         for (int q = 0; q < SPW; ++q) {
             const int i = wv + NW * q;
             if (k * W + i >= nsteps) break;
-            const int d = __builtin_amdgcn_readlane(cur_d.w, q);
+            double(&xv)[R] = (TWO && (q & 1)) ? xvb : xva;
+            const uint32_t dm = (uint32_t)__builtin_amdgcn_readlane(cur_d.w, q);   // steps to wait for
             const int32_t nsh = __builtin_amdgcn_readlane(cur_d.x, q);
             const bool compact = nsh <= WAVE;
             // a first occurrence of a shared pixel puts the window-start value in the pixel's LDS
@@ -334,13 +347,23 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
                     if (win_shared(c) && !(c & WIN_HASPRED)) pub[c & WIN_SLOT_MASK] = xv[u];
                 }
             const int32_t len = __builtin_amdgcn_readlane(cur_d.z, q) - __builtin_amdgcn_readlane(cur_d.y, q);
-            if (d >= 0 && !bailed) {
-                // one LDS access per poll: lanes 0..d read their step's flag, the others the stop flag
+            // LDS addresses of the values earlier steps of the window wrote (-1: none), decoded
+            // before the wait
+            int32_t pa[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int32_t c = code(q, u);
+                pa[u] = win_haspred(c) ? (c & WIN_SLOT_MASK) : -1;
+            }
+            if (dm != 0 && !bailed) {
+                // one LDS access per poll: lane l < W of the mask reads step l's flag, the other
+                // lanes the stop flag
+                const bool mine = lane < W && ((dm >> (lane & 31)) & 1u);
                 int32_t spins = 0;
                 while (true) {
-                    const int32_t v = __hip_atomic_load(&done[lane <= d ? lane : W], __ATOMIC_RELAXED,
+                    const int32_t v = __hip_atomic_load(&done[mine ? lane : W], __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-                    const int32_t f = lane <= d ? v : (v == 0 ? kw : ~kw);
+                    const int32_t f = mine ? v : (v == 0 ? kw : ~kw);
                     if (__builtin_amdgcn_readlane(f, WAVE - 1) != kw) {   // stopped elsewhere
                         bailed = true;
                         break;
@@ -354,14 +377,13 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
                     if (__all(f == kw)) break;
                     ++spins;
                 }
+                __builtin_amdgcn_s_setprio(2);
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
                 if (!bailed) {
                     // each read lands in its slot's register (no temporary), so all are in flight
 #pragma unroll
-                    for (int u = 0; u < R; ++u) {
-                        const int32_t c = code(q, u);
-                        if (win_haspred(c)) xv[u] = pub[c & WIN_SLOT_MASK];
-                    }
+                    for (int u = 0; u < R; ++u)
+                        if (pa[u] >= 0) xv[u] = pub[pa[u]];
                 }
             }
             // the current values of the listed shared entries (lane t: entry t)
@@ -416,7 +438,11 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
                     if (lane + WAVE * u < len && (!win_shared(c) || (c & WIN_LAST))) x[jc[q][u]] = v;
                 }
             }
-            if (q + 1 < SPW && k * W + i + NW < nsteps) gather(q + 1);
+            if (TWO) {
+                if (q + 2 < SPW && k * W + i + 2 * NW < nsteps) gather(q + 2, xv);
+            } else if (q + 1 < SPW && k * W + i + NW < nsteps) {
+                gather(q + 1, xva);
+            }
             if (k + 1 < nwin)
                 load_entries(q, k + 1, __builtin_amdgcn_readlane(nxt_d.y, q), __builtin_amdgcn_readlane(nxt_d.z, q));
         }
@@ -440,66 +466,75 @@ bool art_window_usable(Csr &A, int64_t nsteps) {
     return A.max_row <= 16 * WAVE;
 }
 
-// The window sweep over up to two chains (twin_art's x and y), each with its own sequence.
-void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const int64_t nsteps[2],
-                       double *const x[2], const double *b, const double *nrA, double relax, int lwrb, double lb,
-                       int upb, double ub) {
+namespace {
+
+int win_R(const Csr &A) { return A.max_row <= 8 * WAVE ? 8 : 16; }
+
+}  // namespace
+
+// The pre-pass of one sequence (one-based rows on the device) into buffer set `slot` (0 or 1), on
+// stream s.  (Run beside the previous sweep on a second stream it slowed that sweep more than it
+// saved: uniform randomized Kaczmarz at N = 256, 20 sweeps, 0.42 s against 0.38 s in sequence.)
+void art_window_prepare(Csr &A, int slot, const int32_t *rows, int64_t nsteps, const double *b, const double *nrA,
+                        hipStream_t s) {
     if (A.n_rows_nz < 0) build_row_list(A);
-    const int R = A.max_row <= 8 * WAVE ? 8 : 16;
     FAIR_CHECK(A.max_row <= 16 * WAVE, FAIR_ERR_ARG, "row too long for the window sweep");
+    FAIR_CHECK(slot == 0 || slot == 1, FAIR_ERR_INTERNAL, "window buffer set");
+    const int R = win_R(A);
     ArtWindowBufs &wb = A.win;
     const int64_t n = A.n;
-    // pre-pass tables: 8 bytes per pixel per workgroup, at most 256 MB
+    // pre-pass tables: 8 bytes per pixel per workgroup, at most 128 workgroups (more tables crowd
+    // the matrix out of the caches the sweep then reads it through) and 256 MB
     const int G = (int)std::max<int64_t>(8, std::min<int64_t>(128, ((int64_t)256 << 20) / (8 * n)));
     if (wb.G != G || wb.n != n) {
         wb.mask.alloc((size_t)G * (size_t)n);
         wb.slotof.alloc((size_t)G * (size_t)n);
-        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for((int64_t)G * n, 256)), dim3(256), 0, stream(),
-                           wb.mask.p, (int64_t)G * n);
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for((int64_t)G * n, 256)), dim3(256), 0, s, wb.mask.p,
+                           (int64_t)G * n);
         launch_check("window tables");
         wb.G = G;
         wb.n = n;
     }
+    const size_t need = (size_t)nsteps * (size_t)(WAVE * R);
+    if (wb.codes[slot].n < need) wb.codes[slot].alloc(need);
+    if (wb.desc[slot].n < 4 * (size_t)nsteps) wb.desc[slot].alloc(4 * (size_t)nsteps);
+    if (wb.bw[slot].n < 2 * (size_t)nsteps) wb.bw[slot].alloc(2 * (size_t)nsteps);
+    if (wb.cslot[slot].n < (size_t)nsteps * WAVE) wb.cslot[slot].alloc((size_t)nsteps * WAVE);
+    if (wb.cval[slot].n < (size_t)nsteps * WAVE) wb.cval[slot].alloc((size_t)nsteps * WAVE);
+    wb.nsteps[slot] = nsteps;
+    if (nsteps <= 0) return;
+    const int W = R == 8 ? win_steps<8>() : win_steps<16>();
+    const int64_t nwin = (nsteps + W - 1) / W;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(G, nwin));
+    if (R == 8)
+        hipLaunchKernelGGL(k_win_codes<8>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, s, rows, nsteps, A.IA.p,
+                           A.JA.p, A.vA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p, wb.codes[slot].p,
+                           (int4 *)wb.desc[slot].p, (double2 *)wb.bw[slot].p, wb.cslot[slot].p, wb.cval[slot].p);
+    else
+        hipLaunchKernelGGL(k_win_codes<16>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, s, rows, nsteps, A.IA.p,
+                           A.JA.p, A.vA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p, wb.codes[slot].p,
+                           (int4 *)wb.desc[slot].p, (double2 *)wb.bw[slot].p, wb.cslot[slot].p, wb.cval[slot].p);
+    launch_check("window codes");
+}
+
+// The executor over `chains` chains (chain c: buffer set slots[c], prepared earlier in stream order),
+// iterate x[c] updated in place.  Waits for the sweep and reports a stop.
+void art_window_run(Csr &A, int chains, const int slots[2], double *const x[2], double relax, int lwrb, double lb,
+                    int upb, double ub) {
+    const int R = win_R(A);
+    ArtWindowBufs &wb = A.win;
     if (wb.err.n == 0) wb.err.alloc(1);
     WinChains ch{};
-    for (int c = 0; c < chains; ++c) {
-        const size_t need = (size_t)nsteps[c] * (size_t)(WAVE * R);
-        if (wb.codes[c].n < need) wb.codes[c].alloc(need);
-        if (wb.desc[c].n < 4 * (size_t)nsteps[c]) wb.desc[c].alloc(4 * (size_t)nsteps[c]);
-        if (wb.bw[c].n < 2 * (size_t)nsteps[c]) wb.bw[c].alloc(2 * (size_t)nsteps[c]);
-        if (wb.cslot[c].n < (size_t)nsteps[c] * WAVE) wb.cslot[c].alloc((size_t)nsteps[c] * WAVE);
-        if (wb.cval[c].n < (size_t)nsteps[c] * WAVE) wb.cval[c].alloc((size_t)nsteps[c] * WAVE);
-        ch.nsteps[c] = nsteps[c];
-        ch.x[c] = x[c];
-        ch.codes[c] = wb.codes[c].p;
-        ch.desc[c] = (const int4 *)wb.desc[c].p;
-        ch.bw[c] = (const double2 *)wb.bw[c].p;
-        ch.cslot[c] = wb.cslot[c].p;
-        ch.cval[c] = wb.cval[c].p;
-        const int W = R == 8 ? win_steps<8>() : win_steps<16>();
-        const int64_t nwin = (nsteps[c] + W - 1) / W;
-        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(G, nwin));
-        if (nsteps[c] <= 0) continue;
-        if (R == 8)
-            hipLaunchKernelGGL(k_win_codes<8>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, stream(), rows[c],
-                               nsteps[c], A.IA.p, A.JA.p, A.vA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p,
-                               wb.codes[c].p, (int4 *)wb.desc[c].p, (double2 *)wb.bw[c].p, wb.cslot[c].p,
-                               wb.cval[c].p);
-        else
-            hipLaunchKernelGGL(k_win_codes<16>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, stream(), rows[c],
-                               nsteps[c], A.IA.p, A.JA.p, A.vA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p,
-                               wb.codes[c].p, (int4 *)wb.desc[c].p, (double2 *)wb.bw[c].p, wb.cslot[c].p,
-                               wb.cval[c].p);
-        launch_check("window codes");
-    }
-    for (int c = chains; c < 2; ++c) {
-        ch.nsteps[c] = 0;
-        ch.x[c] = nullptr;
-        ch.codes[c] = nullptr;
-        ch.desc[c] = nullptr;
-        ch.bw[c] = nullptr;
-        ch.cslot[c] = nullptr;
-        ch.cval[c] = nullptr;
+    for (int c = 0; c < 2; ++c) {
+        const bool on = c < chains;
+        const int sl = on ? slots[c] : 0;
+        ch.nsteps[c] = on ? wb.nsteps[sl] : 0;
+        ch.x[c] = on ? x[c] : nullptr;
+        ch.codes[c] = on ? wb.codes[sl].p : nullptr;
+        ch.desc[c] = on ? (const int4 *)wb.desc[sl].p : nullptr;
+        ch.bw[c] = on ? (const double2 *)wb.bw[sl].p : nullptr;
+        ch.cslot[c] = on ? wb.cslot[sl].p : nullptr;
+        ch.cval[c] = on ? wb.cval[sl].p : nullptr;
     }
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream(), (uint32_t *)wb.err.p, (int64_t)1);
     const int32_t spin = std::max(0, env_int("FAIR_ART_WINDOW_SPIN", 1 << 22));
@@ -514,12 +549,20 @@ void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const i
     launch_check("window sweep");
     if (env_int("FAIR_DEBUG", 0))
         fprintf(stderr, "[fair] window sweep: %lld + %lld steps, windows of %d, %d entries per lane\n",
-                (long long)nsteps[0], (long long)(chains > 1 ? nsteps[1] : 0),
-                R == 8 ? win_steps<8>() : win_steps<16>(), R);
+                (long long)ch.nsteps[0], (long long)ch.nsteps[1], R == 8 ? win_steps<8>() : win_steps<16>(), R);
     int32_t herr = 0;
     FAIR_HIP(hipMemcpyAsync(&herr, wb.err.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
     FAIR_HIP(hipStreamSynchronize(stream()));
     FAIR_CHECK(herr == 0, FAIR_ERR_HIP, "Kaczmarz window sweep: a step waited beyond its bound");
+}
+
+// The window sweep over up to two chains (twin_art's x and y), each with its own sequence.
+void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const int64_t nsteps[2],
+                       double *const x[2], const double *b, const double *nrA, double relax, int lwrb, double lb,
+                       int upb, double ub) {
+    const int slots[2] = {0, 1};
+    for (int c = 0; c < chains; ++c) art_window_prepare(A, c, rows[c], nsteps[c], b, nrA, stream());
+    art_window_run(A, chains, slots, x, relax, lwrb, lb, upb, ub);
 }
 
 }  // namespace fair

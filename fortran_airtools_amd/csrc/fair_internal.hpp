@@ -186,6 +186,7 @@ struct ArtWindowBufs {
     DBuf<uint32_t> mask;
     DBuf<int32_t> slotof;
     DBuf<int32_t> err;
+    int64_t nsteps[2] = {0, 0};   // steps each buffer set was prepared for
     int G = 0;
     int64_t n = 0;
 };
@@ -291,6 +292,12 @@ bool art_window_usable(Csr &A, int64_t nsteps);   // FAIR_ART_WINDOW, rows up to
 void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const int64_t nsteps[2],
                        double *const x[2], const double *b, const double *nrA, double relax, int lwrb, double lb,
                        int upb, double ub);
+// Its two halves: the pre-pass into buffer set `slot` (0 or 1) on stream s, and the sweep over
+// prepared sets.
+void art_window_prepare(Csr &A, int slot, const int32_t *rows, int64_t nsteps, const double *b, const double *nrA,
+                        hipStream_t s);
+void art_window_run(Csr &A, int chains, const int slots[2], double *const x[2], double relax, int lwrb, double lb,
+                    int upb, double ub);
 
 // y[r] = sum_k vA[k] * x[JA[k]] over every row; empty rows give 0.
 void spmv(const Csr &A, const double *x, double *y);

@@ -1515,12 +1515,8 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
         scramble(P.nord, uniforms + q, ord.data());
         q += 2 * (int64_t)P.nord;
     }
-    size_t rec = 0;
-    if (exit_iter) *exit_iter = 0;
-    const unsigned nb = grid_for(n, 256);
-    std::shared_ptr<LevelSched> lev;   // held for the whole call
-    for (int iter = 1; iter <= P.maxiter; ++iter) {
-        FAIR_CHECK(q + per_iter <= nu || !P.isrand, FAIR_ERR_ARG, "too few uniforms: need " + std::to_string(need));
+    // the row sequence of one iteration, taking its draws
+    auto build_seq = [&](int32_t *out) {
         if (P.isrand) {                               // :320
             scramble(P.nord, uniforms + q, ord.data());
             q += 2 * (int64_t)P.nord;
@@ -1530,13 +1526,40 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
                 double u = uniforms[q++];
                 auto it = std::lower_bound(P.cumul.begin(), P.cumul.end(), u);
                 int ri = (int)(it - P.cumul.begin()) + 1;
-                seq[ii] = std::min(ri, (int)m);
+                out[ii] = std::min(ri, (int)m);
             } else {
-                seq[ii] = ord[ii];
+                out[ii] = ord[ii];
             }
         }
         if (P.method == FAIR_SYMKACZMARZ)             // :326-328
-            for (int ii = 0; ii < P.nord; ++ii) seq[P.nord + ii] = ord[P.nord - 1 - ii];
+            for (int ii = 0; ii < P.nord; ++ii) out[P.nord + ii] = ord[P.nord - 1 - ii];
+    };
+    size_t rec = 0;
+    if (exit_iter) *exit_iter = 0;
+    const unsigned nb = grid_for(n, 256);
+    // after an iteration: the oracle stop (:330-333) and the recorded iterates (:337-340); true to stop
+    auto after_sweep = [&](int iter) {
+        if (xref) {
+            const double cur = std::sqrt(dev_sum(n, SqDiff{dx.p, dxr.p}, part, out));
+            if (cur > prev) {
+                dxp.download(Xout + (size_t)(nk - 1) * n, n);
+                FAIR_HIP(hipStreamSynchronize(stream()));
+                if (exit_iter) *exit_iter = iter;
+                return true;
+            }
+            prev = cur;
+        }
+        if (rec < P.Kuse.size() && iter == P.Kuse[rec]) {
+            dx.download(Xout + rec * (size_t)n, n);
+            ++rec;
+        }
+        FAIR_HIP(hipStreamSynchronize(stream()));
+        return false;
+    };
+    std::shared_ptr<LevelSched> lev;   // held for the whole call
+    for (int iter = 1; iter <= P.maxiter; ++iter) {
+        FAIR_CHECK(q + per_iter <= nu || !P.isrand, FAIR_ERR_ARG, "too few uniforms: need " + std::to_string(need));
+        build_seq(seq.data());
         if (xref && n > 0) hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, stream(), dx.p, dxp.p, (int64_t)n);
         // without draws every iteration sweeps the same sequence: one level schedule serves all
         if (iter == 1 && !P.isrand) lev = art_levels(h->A, seq.data(), steps_per_iter);
@@ -1547,21 +1570,7 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
             launch_art_chains(h->A, 1, dseq.p, steps_per_iter, dx.p, nullptr, 0, nullptr, db.p, d_nrA.p, relax,
                               box);
         }
-        if (xref) {                                   // :330-333
-            const double cur = std::sqrt(dev_sum(n, SqDiff{dx.p, dxr.p}, part, out));
-            if (cur > prev) {
-                dxp.download(Xout + (size_t)(nk - 1) * n, n);
-                FAIR_HIP(hipStreamSynchronize(stream()));
-                if (exit_iter) *exit_iter = iter;
-                break;
-            }
-            prev = cur;
-        }
-        if (rec < P.Kuse.size() && iter == P.Kuse[rec]) {   // :337-340
-            dx.download(Xout + rec * (size_t)n, n);
-            ++rec;
-        }
-        FAIR_HIP(hipStreamSynchronize(stream()));
+        if (after_sweep(iter)) break;
     }
     if (consumed) *consumed = q;
 }

@@ -1,7 +1,7 @@
 """CPU baseline timing of the reference path (test infrastructure; bench.py's cpu_baseline leg).
 
-Times the reference's own `sirt` ('sart' / 'cimmino'), `gradient_descent` ('gd') or cyclic `art`
-('kaczmarz', serial by construction) (oracle/_ref, the reference's Fortran compiled with flang
+Times the reference's own `sirt` ('sart' / 'cimmino'), `gradient_descent` ('gd'), or `art`, cyclic
+('kaczmarz') or uniform randomized ('randkaczmarz', the runtime's own draws), serial by construction (oracle/_ref, the reference's Fortran compiled with flang
 -O3 -fopenmp; kind "reference") on the host cores, each leg with its own OpenMP thread count, or
 the C restatement (kind "port", single-threaded) when the compiled reference is absent.  Its
 automatic arrays need a stack above 8 MB at N=512, which the caller raises.
@@ -96,7 +96,7 @@ def main() -> None:
         say(f"reference fanlineartomo: {out['reference_build_s']:.2f} s")
     sigma = None
     for name, meth, thr in legs:
-        gd, kz = meth == "gd", meth == "kaczmarz"
+        gd, kz = meth == "gd", meth in ("kaczmarz", "randkaczmarz")
         if kz and not have_ref:
             out["legs"][name] = None
             continue
@@ -108,7 +108,8 @@ def main() -> None:
                 ref.set(0, IA, JA, vA, n)
                 t = time.perf_counter()
                 if kz:                       # k + 1 sweeps: the first carries the set-up
-                    ref.art(0, "kaczmarz", b, [k + 1], lb=0.0, ub=1.0)
+                    ref.rng_seed(1)
+                    ref.art(0, meth, b, [k + 1], lb=0.0, ub=1.0)
                 elif gd:
                     ref.rng_seed(1)          # the same normest start (and iteration count) every run
                     ref.gd(0, b, k, tol=0.0, lb=0.0, ub=1.0)
