@@ -283,8 +283,9 @@ def level_schedule(m: int, n: int, IA, JA, seq):
 def art_steps_dev(A: CSRMatrix, d_rows: int, nsteps: int, d_b: int, d_nrA: int, omg: float, d_x: int,
                   ub: float | None = None, lb: float | None = None) -> None:
     """The row steps of basic_sweep (AIRMs.f90:813-858) with device addresses: int32 one-based rows
-    (nsteps, each in 1..m), b and nrA (m), x (n, updated in place).  Asynchronous on the library
-    stream."""
+    (nsteps, each in 1..m), b and nrA (m), x (n, updated in place).  Returns when the sweep is
+    done, except with the one-wavefront sweep (FAIR_ART_WINDOW=0, or rows over 1,024 entries), which
+    stays queued on the library stream."""
     o = Opts()
     o.relaxpar = float(omg)
     if ub is not None:

@@ -216,12 +216,15 @@ int fair_twin_art_uniform_bound(fair_csr *A, int32_t method, int32_t nk, const i
  * caller makes the row choice: ord, or the row-probability draws).  A sequence with at least 4
  * steps per level of its level schedule (steps of a level touch disjoint pixels) runs level by
  * level; the schedule is cached in the matrix, so repeating one sequence plans it once.  Other
- * sequences run as one persistent launch.  The iterate is the same bit for bit either way.
+ * sequences run in the window sweep (one workgroup; steps that share no pixel run side by side).
+ * The iterate is the same bit for bit either way.
  * nrA: the rows' (damped squared) norms (m); x (n) updated in place.  Host arrays; rows outside
  * 1..m are rejected. */
 int fair_art_steps(fair_csr *A, const int32_t *rows, int64_t nsteps, const double *b, const double *nrA,
                    const fair_opts *opts, double *x);
-/* The same with device arrays (d_x in HBM), asynchronous; the rows must lie in 1..m. */
+/* The same with device arrays (d_x in HBM); the rows must lie in 1..m.  Returns when the sweep is
+ * done (the window sweep reads its stop flag back); the one-wavefront sweep (FAIR_ART_WINDOW=0, or
+ * rows over 1,024 entries) leaves it running on the library stream. */
 int fair_art_steps_dev(fair_csr *A, const int32_t *d_rows, int64_t nsteps, const double *d_b,
                        const double *d_nrA, const fair_opts *opts, double *d_x);
 /* The level schedule the sweeps use, on host arrays (no device call).  No counterpart in the
