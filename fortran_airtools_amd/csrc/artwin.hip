@@ -10,17 +10,21 @@
 // steps that do not conflict run at the same time.
 //
 // A pre-pass kernel classifies each entry of each step (per window, from a bit mask of the steps
-// touching each pixel):
-//   0xFFFF                     the pixel is touched by no other step of the window: the step
-//                              uses the window-start value and stores its result to x;
-//   slot | P << 14 | L << 15   shared: P = an earlier step of the window touched it (read the
-//                              value from LDS slot `slot`, else use the window-start value),
-//                              L = no later step touches it (store the result to x);
+// touching each pixel) into a 16-bit code, idx | P << 14 | L << 15, where idx is the LDS index
+// the entry's value is read from once the step's wait is over (PUB: the number of shared slots):
+//   idx >= PUB   the pixel is touched by no other step of the window: idx is the entry's own
+//                parking place (wavefront, register slot, lane), where the step puts its
+//                window-start value before the wait; it stores its result to x;
+//   idx <  PUB   shared: idx is the pixel's LDS slot.  P = an earlier step of the window touched
+//                it (the slot then holds that step's value), else the step itself puts the
+//                window-start value there; L = no later step touches it (store the result to x);
 // and records per step the mask of the earlier steps of its window that share a pixel with it.
 // The executor then computes every row exactly as k_art_sweep does (entries of lane q:
 // beg + q + 64 t in increasing t, products and sums rounded separately, the DPP wavefront sum,
 // coeff = relax*(b - dot)/nrA, x + coeff*a then max(lb), min(ub)), on the same values in the
-// same order, so its iterate equals k_art_sweep's bit for bit.
+// same order, so its iterate equals k_art_sweep's bit for bit.  Columns outside the image (the
+// parallel-beam tracer emits some above N^2 when an angle is exactly pi) read zero and are not
+// stored, as in k_art_sweep.
 #include <algorithm>
 #include <cstdlib>
 
@@ -29,21 +33,29 @@
 namespace fair {
 namespace {
 
-constexpr uint16_t WIN_UNSHARED = 0xFFFF;
-constexpr int WIN_SLOT_MASK = 0x1FFF;
+constexpr int WIN_IDX_MASK = 0x3FFF;
 constexpr int WIN_HASPRED = 1 << 14;
 constexpr int WIN_LAST = 1 << 15;
 constexpr int WIN_PRE_WAVES = 8;   // pre-pass workgroup: 512 threads
 
 // Window sizes: W steps, NW wavefronts of SPW steps each (wavefront v: steps v, v + NW, ...).
-// Rows up to 64 R entries.  LDS holds a double per shared pixel: at most half of a window's
-// entries, W * 64 R / 2 (64 KB for both shapes).
+// Rows up to 64 R entries.  The executor's LDS holds PUB = W * 64 R / 2 doubles, one per shared
+// pixel (at most half of a window's entries), and a parking place per wavefront, register slot and
+// lane: 96 KB for both shapes.
 template <int R> struct WinShape;
 template <> struct WinShape<8> { static constexpr int NW = 8, SPW = 4; };
 template <> struct WinShape<16> { static constexpr int NW = 8, SPW = 1; };
 
 template <int R> constexpr int win_steps() { return WinShape<R>::NW * WinShape<R>::SPW; }
 template <int R> constexpr int win_pub() { return win_steps<R>() * WAVE * R / 2; }
+// the executor's LDS: the shared slots, then the parking places (NW wavefronts x R slots x 64 lanes)
+template <int R> constexpr int win_lds() { return win_pub<R>() + WinShape<R>::NW * R * WAVE; }
+static_assert(win_lds<8>() <= WIN_IDX_MASK + 1 && win_lds<16>() <= WIN_IDX_MASK + 1, "LDS index field");
+// the parking place of lane `lane`, register slot u, of the wavefront that runs window step i
+template <int R>
+__device__ __forceinline__ int32_t win_park(int i, int u, int lane) {
+    return win_pub<R>() + ((i % WinShape<R>::NW) * R + u) * WAVE + lane;
+}
 
 struct WinChains {
     int64_t nsteps[2];
@@ -158,12 +170,12 @@ __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_
             for (int u = 0; u < R; ++u) {
                 const int32_t j = jc[q][u];
                 const uint32_t M = mk[q][u];
-                uint16_t c = WIN_UNSHARED;
+                uint16_t c = (uint16_t)win_park<R>(i, u, lane);
                 int32_t sl = 0;
                 const bool sh = __popc(M) >= 2;
                 if (sh) {
                     sl = __hip_atomic_load(&slotof[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    c = (uint16_t)((sl & WIN_SLOT_MASK) | ((M & ((1u << i) - 1u)) ? WIN_HASPRED : 0) |
+                    c = (uint16_t)((sl & WIN_IDX_MASK) | ((M & ((1u << i) - 1u)) ? WIN_HASPRED : 0) |
                                    ((M >> i) == 1u ? WIN_LAST : 0));
                 }
                 codes[s * (WAVE * R) + lane * R + u] = c;
@@ -172,7 +184,7 @@ __global__ void __launch_bounds__(WIN_PRE_WAVES * WAVE) k_win_codes(const int32_
                     nsh + (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
                 if (sh && pos < WAVE) {
-                    cslot[s * WAVE + pos] = sl & WIN_SLOT_MASK;
+                    cslot[s * WAVE + pos] = sl & WIN_IDX_MASK;
                     cval[s * WAVE + pos] = vA[bq[q] + lane + WAVE * u];
                 }
                 nsh += __popcll(bal);
@@ -219,8 +231,7 @@ __device__ __forceinline__ double win_wave_sum(double v) {
 // The value a padded slot gathers (its product must be +0 whatever x holds).
 __device__ double g_win_zero = 0.0;
 
-__device__ __forceinline__ bool win_shared(int32_t c) { return c != WIN_UNSHARED; }
-__device__ __forceinline__ bool win_haspred(int32_t c) { return c != WIN_UNSHARED && (c & WIN_HASPRED); }
+template <int R> __device__ __forceinline__ bool win_shared(int32_t c) { return (c & WIN_IDX_MASK) < win_pub<R>(); }
 
 // ---- executor ----------------------------------------------------------------------------------
 // One workgroup per chain.  Each wavefront holds the entries and codes of its SPW steps of the
@@ -241,10 +252,12 @@ __device__ __forceinline__ bool win_haspred(int32_t c) { return c != WIN_UNSHARE
 template <int R>
 __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains ch, const int32_t *__restrict__ JA,
                                                                      const double *__restrict__ vA, int32_t pad,
-                                                                     double relax, int lwrb, double lb, int upb,
+                                                                     int32_t ncols, double relax, int lwrb, double lb, int upb,
                                                                      double ub, int32_t spin_limit, int32_t *err) {
-    constexpr int NW = WinShape<R>::NW, SPW = WinShape<R>::SPW, W = NW * SPW, PUB = win_pub<R>();
-    __shared__ double pub[PUB];
+    constexpr int NW = WinShape<R>::NW, SPW = WinShape<R>::SPW, W = NW * SPW;
+    // pub: a double per shared pixel; after it, per wavefront and register slot, the 64 lanes'
+    // window-start values of the step being waited for (the patch reads take every slot from LDS)
+    __shared__ double pub[win_lds<R>()];
     __shared__ int32_t done[W + 1];   // done[W]: the stop flag (0 while running)
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -259,15 +272,19 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
     __syncthreads();
     const int64_t nwin = (nsteps + W - 1) / W;
     if (nwin == 0) return;   // the whole workgroup leaves together
-    int32_t jc[SPW][R];
-    uint32_t cw[SPW][R / 2];   // the lane's codes, two per word
-    double vc[SPW][R];
+    // The entries of a step are held in register slot q % P from two turns of the wavefront before
+    // its own (P = 2, when SPW is even; else one window ahead, P = SPW): the registers then do not
+    // grow with the window.
+    constexpr int P = SPW % 2 == 0 ? 2 : SPW;
+    int32_t jc[P][R];
+    uint32_t cw[P][R / 2];   // the lane's codes, two per word
+    double vc[P][R];
     // x at the pixels of the next step, or of the next two (by parity of q) where the registers
     // allow (three steps or fewer per wavefront)
     constexpr bool TWO = SPW <= 3;
     double xva[R], xvb[R];
-    int32_t ccs[SPW];          // lane t: LDS slot and matrix value of the step's shared entry t
-    double cva[SPW];
+    int32_t ccs[P];            // lane t: LDS slot and matrix value of the step's shared entry t
+    double cva[P];
     // Descriptors of the wavefront's steps, lane q holding step q: {beg, end, dep} and {b, nrA} of
     // this window (cur) and of the next (nxt), loaded a window ahead by one vector load each.
     int4 cur_d, nxt_d;
@@ -276,7 +293,9 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
     // the box bounds live in vector registers: a select then needs no scalar-to-vector move
     double lbv = lb, ubv = ub;
     asm volatile("" : "+v"(lbv), "+v"(ubv));
-    auto code = [&](int q, int u) -> int32_t { return (int32_t)((cw[q][u >> 1] >> ((u & 1) * 16)) & 0xFFFFu); };
+    auto code = [&](int q, int u) -> int32_t {
+        return (int32_t)((cw[q % P][u >> 1] >> ((u & 1) * 16)) & 0xFFFFu);
+    };
     // lanes 0..SPW-1: the descriptors of the wavefront's steps of window k (a step past the end: no
     // entries, no wait)
     auto load_descs = [&](int64_t k, int4 &d, double2 &bv) {
@@ -290,31 +309,31 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
     // the zero spare entry `pad` after the matrix's entries (column 0, value 0): no load result is
     // selected afterwards, so the loads stay in flight together; a slot's validity is tested
     // where it matters (lane + 64 u < the row's length).
-    auto load_entries = [&](int q, int64_t k, int32_t beg, int32_t end) {
+    auto load_entries = [&](int q, int64_t k, int32_t beg, int32_t end) {   // step q of window k
         const int64_t s = k * W + wv + NW * q;
         const int64_t sc = s < nsteps ? s : 0;
 #pragma unroll
         for (int u = 0; u < R; ++u) {
             const int32_t e = beg + lane + WAVE * u;
             const int32_t ee = e < end ? e : pad;
-            jc[q][u] = JA[ee];
-            vc[q][u] = vA[ee];
+            jc[q % P][u] = JA[ee];
+            vc[q % P][u] = vA[ee];
         }
-        ccs[q] = cslot[sc * WAVE + lane];
-        cva[q] = cval[sc * WAVE + lane];
+        ccs[q % P] = cslot[sc * WAVE + lane];
+        cva[q % P] = cval[sc * WAVE + lane];
         const uint4 *cp = reinterpret_cast<const uint4 *>(codes + sc * (WAVE * R) + lane * R);
 #pragma unroll
         for (int h = 0; h < R / 8; ++h) {
             const uint4 c4 = cp[h];
-            cw[q][4 * h] = c4.x;
-            cw[q][4 * h + 1] = c4.y;
-            cw[q][4 * h + 2] = c4.z;
-            cw[q][4 * h + 3] = c4.w;
+            cw[q % P][4 * h] = c4.x;
+            cw[q % P][4 * h + 1] = c4.y;
+            cw[q % P][4 * h + 2] = c4.z;
+            cw[q % P][4 * h + 3] = c4.w;
         }
     };
     load_descs(0, cur_d, cur_b);
 #pragma unroll
-    for (int q = 0; q < SPW; ++q)
+    for (int q = 0; q < P; ++q)
         load_entries(q, 0, __builtin_amdgcn_readlane(cur_d.y, q), __builtin_amdgcn_readlane(cur_d.z, q));
     if (nwin > 1) load_descs(1, nxt_d, nxt_b);
     for (int64_t k = 0; k < nwin; ++k) {
@@ -326,7 +345,8 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
         auto gather = [&](int q, double (&dst)[R]) {
             const int32_t len = __builtin_amdgcn_readlane(cur_d.z, q) - __builtin_amdgcn_readlane(cur_d.y, q);
 #pragma unroll
-            for (int u = 0; u < R; ++u) dst[u] = *(lane + WAVE * u < len ? x + jc[q][u] : &g_win_zero);
+            for (int u = 0; u < R; ++u)
+                dst[u] = *(lane + WAVE * u < len && (uint32_t)jc[q % P][u] < (uint32_t)ncols ? x + jc[q % P][u] : &g_win_zero);
         };
         gather(0, xva);
         if (TWO && SPW > 1) gather(1, xvb);
@@ -344,107 +364,132 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
 #pragma unroll
                 for (int u = 0; u < R; ++u) {
                     const int32_t c = code(q, u);
-                    if (win_shared(c) && !(c & WIN_HASPRED)) pub[c & WIN_SLOT_MASK] = xv[u];
+                    if (win_shared<R>(c) && !(c & WIN_HASPRED)) pub[c & WIN_IDX_MASK] = xv[u];
                 }
             const int32_t len = __builtin_amdgcn_readlane(cur_d.z, q) - __builtin_amdgcn_readlane(cur_d.y, q);
-            // LDS addresses of the values earlier steps of the window wrote (-1: none), decoded
-            // before the wait
-            int32_t pa[R];
-#pragma unroll
-            for (int u = 0; u < R; ++u) {
-                const int32_t c = code(q, u);
-                pa[u] = win_haspred(c) ? (c & WIN_SLOT_MASK) : -1;
-            }
-            if (dm != 0 && !bailed) {
-                // one LDS access per poll: lane l < W of the mask reads step l's flag, the other
-                // lanes the stop flag
-                const bool mine = lane < W && ((dm >> (lane & 31)) & 1u);
-                int32_t spins = 0;
-                while (true) {
-                    const int32_t v = __hip_atomic_load(&done[mine ? lane : W], __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                    const int32_t f = mine ? v : (v == 0 ? kw : ~kw);
-                    if (__builtin_amdgcn_readlane(f, WAVE - 1) != kw) {   // stopped elsewhere
-                        bailed = true;
-                        break;
-                    }
-                    if (spins >= spin_limit) {
-                        __hip_atomic_store(&done[W], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (lane == 0) atomicExch(err, 1);
-                        bailed = true;
-                        break;
-                    }
-                    if (__all(f == kw)) break;
-                    ++spins;
-                }
-                __builtin_amdgcn_s_setprio(2);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-                if (!bailed) {
-                    // each read lands in its slot's register (no temporary), so all are in flight
-#pragma unroll
-                    for (int u = 0; u < R; ++u)
-                        if (pa[u] >= 0) xv[u] = pub[pa[u]];
-                }
-            }
-            // the current values of the listed shared entries (lane t: entry t)
-            const double cv = pub[lane < nsh ? ccs[q] : 0];
-            if (!bailed) {
+            // register slots holding entries of the row (wave-uniform); the slots past them hold
+            // zero products only, which the dot product leaves out
+            const int nsl = (len + WAVE - 1) / WAVE;
+            // The step's turn on its row values xw: the lane sums (k_art_sweep's order; the slots
+            // past nsl hold +0 products, which change no sum, and are left out), the wavefront sum,
+            // coeff, the shared pixels published and the step marked done, then the stores.
+            auto turn = [&](double(&xw)[R]) {
+                // the current values of the listed shared entries (lane t: entry t)
+                const double cv = pub[lane < nsh ? ccs[q % P] : 0];
                 __builtin_amdgcn_s_setprio(2);
                 double dot = 0.0;
 #pragma unroll
-                for (int u = 0; u < R; ++u) dot += xv[u] * vc[q][u];
+                for (int g = 0; g < R / 4; ++g) {
+                    if (g > 0 && 4 * g >= nsl) break;
+#pragma unroll
+                    for (int u = 4 * g; u < 4 * g + 4; ++u) dot += xw[u] * vc[q % P][u];
+                }
                 dot = win_wave_sum(dot);
                 const double coeff = (relax * (lane_value(cur_b.x, q) - dot)) / lane_value(cur_b.y, q);
-                auto update_all = [&] {
-#pragma unroll
-                    for (int u = 0; u < R; ++u) {
-                        double v = xv[u] + coeff * vc[q][u];
-                        if (lwrb) v = fmax_ref(v, lbv);
-                        if (upb) v = fmin_ref(v, ubv);
-                        xv[u] = v;
-                    }
-                };
                 if (compact) {
                     // only the shared pixels are needed before the step is marked done
                     if (lane < nsh) {
-                        double v = cv + coeff * cva[q];
+                        double v = cv + coeff * cva[q % P];
                         if (lwrb) v = fmax_ref(v, lbv);
                         if (upb) v = fmin_ref(v, ubv);
-                        pub[ccs[q]] = v;
+                        pub[ccs[q % P]] = v;
                     }
                 } else {
-                    update_all();
 #pragma unroll
                     for (int u = 0; u < R; ++u) {
+                        if (u >= nsl) break;
+                        double v = xw[u] + coeff * vc[q % P][u];
+                        if (lwrb) v = fmax_ref(v, lbv);
+                        if (upb) v = fmin_ref(v, ubv);
+                        xw[u] = v;
                         const int32_t c = code(q, u);
-                        if (win_shared(c)) pub[c & WIN_SLOT_MASK] = xv[u];
+                        if (win_shared<R>(c)) pub[c & WIN_IDX_MASK] = v;
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
                 if (lane == 0) __hip_atomic_store(&done[i], kw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 __builtin_amdgcn_s_setprio(0);
-                // slots past the row (u >= nsl) hold no entry in any lane: skipped by a uniform test
-                const int nsl = (len + WAVE - 1) / WAVE;
+                // slots past the row (u >= nsl) hold no entry in any lane: skipped by a uniform test;
+                // columns outside the image take no store
 #pragma unroll
                 for (int u = 0; u < R; ++u) {
                     if (u >= nsl) break;
-                    double v = xv[u];
+                    double v = xw[u];
                     if (compact) {
-                        v = v + coeff * vc[q][u];
+                        v = v + coeff * vc[q % P][u];
                         if (lwrb) v = fmax_ref(v, lbv);
                         if (upb) v = fmin_ref(v, ubv);
                     }
                     const int32_t c = code(q, u);
-                    if (lane + WAVE * u < len && (!win_shared(c) || (c & WIN_LAST))) x[jc[q][u]] = v;
+                    if (lane + WAVE * u < len && (uint32_t)jc[q % P][u] < (uint32_t)ncols &&
+                        (!win_shared<R>(c) || (c & WIN_LAST)))
+                        x[jc[q % P][u]] = v;
+                }
+            };
+            bool waited = false;
+            if (dm != 0 && !bailed) {
+                // every slot's window-start value goes to its parking place, so that after the wait
+                // each slot reads its value from LDS at the index its code holds, with no select
+#pragma unroll
+                for (int g = 0; g < R / 4; ++g) {
+                    if (g > 0 && 4 * g >= nsl) break;
+#pragma unroll
+                    for (int u = 4 * g; u < 4 * g + 4; ++u) pub[win_park<R>(wv, u, lane)] = xv[u];
+                }
+                // one LDS access per poll: lane l < W of the mask reads step l's flag, the other
+                // lanes the stop flag (lane 63 among them)
+                const bool mine = lane < W && ((dm >> (lane & 31)) & 1u);
+                const int32_t want = mine ? kw : 0;
+                if (spin_limit <= 0) {
+                    __hip_atomic_store(&done[W], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (lane == 0) atomicExch(err, 1);
+                    bailed = true;
+                } else {
+                    int32_t spins = 0;
+                    while (true) {
+                        const int32_t v = __hip_atomic_load(&done[mine ? lane : W], __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (__all(v == want)) break;
+                        if (__builtin_amdgcn_readlane(v, WAVE - 1) != 0) {   // stopped elsewhere
+                            bailed = true;
+                            break;
+                        }
+                        if (++spins >= spin_limit) {
+                            __hip_atomic_store(&done[W], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (lane == 0) atomicExch(err, 1);
+                            bailed = true;
+                            break;
+                        }
+                    }
+                }
+                __builtin_amdgcn_s_setprio(2);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                waited = !bailed;
+            }
+            if (waited) {
+                // every slot read back from LDS, the reads issued back to back
+#pragma unroll
+                for (int g = 0; g < R / 4; ++g) {
+                    if (g > 0 && 4 * g >= nsl) break;
+#pragma unroll
+                    for (int u = 4 * g; u < 4 * g + 4; ++u) xv[u] = pub[code(q, u) & WIN_IDX_MASK];
                 }
             }
+            if (!bailed) turn(xv);
             if (TWO) {
                 if (q + 2 < SPW && k * W + i + 2 * NW < nsteps) gather(q + 2, xv);
             } else if (q + 1 < SPW && k * W + i + NW < nsteps) {
                 gather(q + 1, xva);
             }
-            if (k + 1 < nwin)
-                load_entries(q, k + 1, __builtin_amdgcn_readlane(nxt_d.y, q), __builtin_amdgcn_readlane(nxt_d.z, q));
+            // the entries of the wavefront's step P turns ahead, into the slot this step held
+            // (loading the next window's past the barrier instead delays the window's first steps:
+            // measured slower)
+            if (q + P < SPW)
+                load_entries(q + P, k, __builtin_amdgcn_readlane(cur_d.y, q + P),
+                             __builtin_amdgcn_readlane(cur_d.z, q + P));
+            else if (k + 1 < nwin)
+                load_entries(q + P - SPW, k + 1, __builtin_amdgcn_readlane(nxt_d.y, q + P - SPW),
+                             __builtin_amdgcn_readlane(nxt_d.z, q + P - SPW));
         }
         __syncthreads();
         cur_d = nxt_d;
@@ -542,10 +587,10 @@ void art_window_run(Csr &A, int chains, const int slots[2], double *const x[2], 
     const int32_t pad = (int32_t)A.nnz;
     if (R == 8)
         hipLaunchKernelGGL(k_art_window<8>, dim3(chains), dim3(WinShape<8>::NW * WAVE), 0, stream(), ch, A.JA.p,
-                           A.vA.p, pad, relax, lwrb, lb, upb, ub, spin, wb.err.p);
+                           A.vA.p, pad, A.n, relax, lwrb, lb, upb, ub, spin, wb.err.p);
     else
         hipLaunchKernelGGL(k_art_window<16>, dim3(chains), dim3(WinShape<16>::NW * WAVE), 0, stream(), ch, A.JA.p,
-                           A.vA.p, pad, relax, lwrb, lb, upb, ub, spin, wb.err.p);
+                           A.vA.p, pad, A.n, relax, lwrb, lb, upb, ub, spin, wb.err.p);
     launch_check("window sweep");
     if (env_int("FAIR_DEBUG", 0))
         fprintf(stderr, "[fair] window sweep: %lld + %lld steps, windows of %d, %d entries per lane\n",
