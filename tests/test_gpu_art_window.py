@@ -10,7 +10,7 @@ iterates must be equal, not merely close.  Covered:
 - basic_sweep on explicit sequences around the window length (32 steps): short ones, a row
   repeated, rows repeated in pairs (every pixel shared), adjacent rays, forward and back, random
   draws with replacement;
-- rows above 512 entries (16 entries per lane, windows of 16 steps);
+- rows above 512 entries (16 entries per lane, windows of 8 steps);
 - the N = 256 fan of the Kaczmarz benchmark;
 - the device-address entry point;
 - the bounded wait: with a bound of zero polls the call fails with an error and the next call is
@@ -154,7 +154,7 @@ def test_basic_sweep_sequences(monkeypatch, capfd, oracle):
 
 
 def test_long_rows(monkeypatch, capfd, oracle):
-    """Rows of 600 and 1000 entries: 16 entries per lane, windows of 16 steps."""
+    """Rows of 600 and 1000 entries: 16 entries per lane, windows of 8 steps."""
     _, sp, am = fair()
     rng = np.random.default_rng(3)
     m, n = 300, 2048
