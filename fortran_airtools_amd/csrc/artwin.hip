@@ -26,6 +26,7 @@
 // parallel-beam tracer emits some above N^2 when an angle is exactly pi) read zero and are not
 // stored, as in k_art_sweep.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 
 #include "fair_internal.hpp"
@@ -312,12 +313,26 @@ __global__ void __launch_bounds__(WinShape<R>::NW *WAVE) k_art_window(WinChains 
     auto load_entries = [&](int q, int64_t k, int32_t beg, int32_t end) {   // step q of window k
         const int64_t s = k * W + wv + NW * q;
         const int64_t sc = s < nsteps ? s : 0;
+        // with 16 slots per lane the slots past the row's end are not loaded (they would all read the
+        // spare entry; measured faster at N=512); with 8, where rows fill most slots, the test costs
+        // more than it saves
 #pragma unroll
-        for (int u = 0; u < R; ++u) {
-            const int32_t e = beg + lane + WAVE * u;
-            const int32_t ee = e < end ? e : pad;
-            jc[q % P][u] = JA[ee];
-            vc[q % P][u] = vA[ee];
+        for (int g = 0; g < R / 4; ++g) {
+            if (R == 8 || g == 0 || WAVE * 4 * g < end - beg) {
+#pragma unroll
+                for (int u = 4 * g; u < 4 * g + 4; ++u) {
+                    const int32_t e = beg + lane + WAVE * u;
+                    const int32_t ee = e < end ? e : pad;
+                    jc[q % P][u] = JA[ee];
+                    vc[q % P][u] = vA[ee];
+                }
+            } else {
+#pragma unroll
+                for (int u = 4 * g; u < 4 * g + 4; ++u) {
+                    jc[q % P][u] = 0;
+                    vc[q % P][u] = 0.0;
+                }
+            }
         }
         ccs[q % P] = cslot[sc * WAVE + lane];
         cva[q % P] = cval[sc * WAVE + lane];
@@ -503,6 +518,43 @@ __global__ void k_zero_u32(uint32_t *p, int64_t n) {
     if (i < n) p[i] = 0u;
 }
 
+// ---- the window sweep's pixel numbering in 4 x 4 tiles ---------------------------------------
+// Pixel j = c N + r of an N x N image (c, r < N) becomes ((c/4) (NP/4) + r/4) 16 + (c%4) 4 + r%4,
+// NP = N rounded up to a multiple of 4: the 16 pixels of a tile are consecutive (two 128-byte
+// lines), so a ray's consecutive pixels, in whatever direction it runs, share lines: at N = 256 a
+// row step's gathers touch 66 lines instead of 135 in the image's own order.  The sweep runs on
+// the renumbered columns and a renumbered copy of x (copied exactly, in and out), so its iterate is
+// the same, bit for bit.
+__host__ __device__ __forceinline__ int32_t win_tiled(int32_t j, int32_t N, int32_t NP) {
+    const int32_t c = j / N, r = j - c * N;
+    return ((c >> 2) * (NP >> 2) + (r >> 2)) * 16 + (c & 3) * 4 + (r & 3);
+}
+
+// columns of the matrix's entries (and its spare entries, column 0) renumbered; columns outside
+// the image become INT32_MAX (outside the renumbered image too)
+__global__ void k_tile_cols(const int32_t *__restrict__ JA, int32_t *__restrict__ JAt, int64_t total, int32_t n,
+                            int32_t N, int32_t NP) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int32_t j = JA[e];
+    JAt[e] = (uint32_t)j < (uint32_t)n ? win_tiled(j, N, NP) : INT32_MAX;
+}
+
+// xt[t] = x at the pixel renumbered t (0 for the padding pixels), for t < NP^2
+__global__ void k_x_to_tiles(const double *__restrict__ x, double *__restrict__ xt, int32_t N, int32_t NP) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)NP * NP) return;
+    const int32_t tile = (int32_t)(t >> 4), w = (int32_t)(t & 15);
+    const int32_t c = (tile / (NP >> 2)) * 4 + (w >> 2), r = (tile % (NP >> 2)) * 4 + (w & 3);
+    xt[t] = c < N && r < N ? x[(int64_t)c * N + r] : 0.0;
+}
+
+__global__ void k_x_from_tiles(const double *__restrict__ xt, double *__restrict__ x, int32_t N, int32_t NP) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int64_t)N * N) return;
+    x[j] = xt[win_tiled((int32_t)j, N, NP)];
+}
+
 }  // namespace
 
 bool art_window_usable(Csr &A, int64_t nsteps) {
@@ -520,14 +572,14 @@ int win_R(const Csr &A) { return A.max_row <= 8 * WAVE ? 8 : 16; }
 // The pre-pass of one sequence (one-based rows on the device) into buffer set `slot` (0 or 1), on
 // stream s.  (Run beside the previous sweep on a second stream it slowed that sweep more than it
 // saved: uniform randomized Kaczmarz at N = 256, 20 sweeps, 0.42 s against 0.38 s in sequence.)
-void art_window_prepare(Csr &A, int slot, const int32_t *rows, int64_t nsteps, const double *b, const double *nrA,
-                        hipStream_t s) {
+void art_window_prepare(Csr &A, const int32_t *cols, int32_t ncols, int slot, const int32_t *rows, int64_t nsteps,
+                        const double *b, const double *nrA, hipStream_t s) {
     if (A.n_rows_nz < 0) build_row_list(A);
     FAIR_CHECK(A.max_row <= 16 * WAVE, FAIR_ERR_ARG, "row too long for the window sweep");
     FAIR_CHECK(slot == 0 || slot == 1, FAIR_ERR_INTERNAL, "window buffer set");
     const int R = win_R(A);
     ArtWindowBufs &wb = A.win;
-    const int64_t n = A.n;
+    const int64_t n = ncols;
     // pre-pass tables: 8 bytes per pixel per workgroup, at most 128 workgroups (more tables crowd
     // the matrix out of the caches the sweep then reads it through) and 256 MB
     const int G = (int)std::max<int64_t>(8, std::min<int64_t>(128, ((int64_t)256 << 20) / (8 * n)));
@@ -553,19 +605,19 @@ void art_window_prepare(Csr &A, int slot, const int32_t *rows, int64_t nsteps, c
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(G, nwin));
     if (R == 8)
         hipLaunchKernelGGL(k_win_codes<8>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, s, rows, nsteps, A.IA.p,
-                           A.JA.p, A.vA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p, wb.codes[slot].p,
+                           cols, A.vA.p, b, nrA, ncols, wb.mask.p, wb.slotof.p, wb.codes[slot].p,
                            (int4 *)wb.desc[slot].p, (double2 *)wb.bw[slot].p, wb.cslot[slot].p, wb.cval[slot].p);
     else
         hipLaunchKernelGGL(k_win_codes<16>, dim3(grid), dim3(WIN_PRE_WAVES * WAVE), 0, s, rows, nsteps, A.IA.p,
-                           A.JA.p, A.vA.p, b, nrA, A.n, wb.mask.p, wb.slotof.p, wb.codes[slot].p,
+                           cols, A.vA.p, b, nrA, ncols, wb.mask.p, wb.slotof.p, wb.codes[slot].p,
                            (int4 *)wb.desc[slot].p, (double2 *)wb.bw[slot].p, wb.cslot[slot].p, wb.cval[slot].p);
     launch_check("window codes");
 }
 
 // The executor over `chains` chains (chain c: buffer set slots[c], prepared earlier in stream order),
 // iterate x[c] updated in place.  Waits for the sweep and reports a stop.
-void art_window_run(Csr &A, int chains, const int slots[2], double *const x[2], double relax, int lwrb, double lb,
-                    int upb, double ub) {
+void art_window_run(Csr &A, const int32_t *cols, int32_t ncols, int chains, const int slots[2], double *const x[2],
+                    double relax, int lwrb, double lb, int upb, double ub) {
     const int R = win_R(A);
     ArtWindowBufs &wb = A.win;
     if (wb.err.n == 0) wb.err.alloc(1);
@@ -586,11 +638,11 @@ void art_window_run(Csr &A, int chains, const int slots[2], double *const x[2], 
     // the first spare entry after the matrix's entries (JA = 0, vA = 0, see alloc_entries)
     const int32_t pad = (int32_t)A.nnz;
     if (R == 8)
-        hipLaunchKernelGGL(k_art_window<8>, dim3(chains), dim3(WinShape<8>::NW * WAVE), 0, stream(), ch, A.JA.p,
-                           A.vA.p, pad, A.n, relax, lwrb, lb, upb, ub, spin, wb.err.p);
+        hipLaunchKernelGGL(k_art_window<8>, dim3(chains), dim3(WinShape<8>::NW * WAVE), 0, stream(), ch, cols,
+                           A.vA.p, pad, ncols, relax, lwrb, lb, upb, ub, spin, wb.err.p);
     else
-        hipLaunchKernelGGL(k_art_window<16>, dim3(chains), dim3(WinShape<16>::NW * WAVE), 0, stream(), ch, A.JA.p,
-                           A.vA.p, pad, A.n, relax, lwrb, lb, upb, ub, spin, wb.err.p);
+        hipLaunchKernelGGL(k_art_window<16>, dim3(chains), dim3(WinShape<16>::NW * WAVE), 0, stream(), ch, cols,
+                           A.vA.p, pad, ncols, relax, lwrb, lb, upb, ub, spin, wb.err.p);
     launch_check("window sweep");
     if (env_int("FAIR_DEBUG", 0))
         fprintf(stderr, "[fair] window sweep: %lld + %lld steps, windows of %d, %d entries per lane\n",
@@ -601,13 +653,60 @@ void art_window_run(Csr &A, int chains, const int slots[2], double *const x[2], 
     FAIR_CHECK(herr == 0, FAIR_ERR_HIP, "Kaczmarz window sweep: a step waited beyond its bound");
 }
 
-// The window sweep over up to two chains (twin_art's x and y), each with its own sequence.
+namespace {
+
+// The side N of the square image whose pixels the matrix's columns are (n = N^2, N >= 16), for the
+// tiled numbering; 0 when the columns do not form one or FAIR_ART_TILED=0.
+int32_t win_tile_side(const Csr &A) {
+    if (!env_int("FAIR_ART_TILED", 1) || A.n < 256) return 0;
+    int64_t N = (int64_t)std::sqrt((double)A.n);
+    while (N * N > A.n) --N;
+    while ((N + 1) * (N + 1) <= A.n) ++N;
+    return N * N == A.n ? (int32_t)N : 0;
+}
+
+}  // namespace
+
+// The window sweep over up to two chains (twin_art's x and y), each with its own sequence.  For a
+// square image it runs on the tiled numbering: the renumbered columns are made once per matrix, the
+// iterates are copied into it and back around the sweep.
 void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const int64_t nsteps[2],
                        double *const x[2], const double *b, const double *nrA, double relax, int lwrb, double lb,
                        int upb, double ub) {
     const int slots[2] = {0, 1};
-    for (int c = 0; c < chains; ++c) art_window_prepare(A, c, rows[c], nsteps[c], b, nrA, stream());
-    art_window_run(A, chains, slots, x, relax, lwrb, lb, upb, ub);
+    const int32_t N = win_tile_side(A);
+    if (N == 0) {
+        for (int c = 0; c < chains; ++c) art_window_prepare(A, A.JA.p, A.n, c, rows[c], nsteps[c], b, nrA, stream());
+        art_window_run(A, A.JA.p, A.n, chains, slots, x, relax, lwrb, lb, upb, ub);
+        return;
+    }
+    const int32_t NP = (N + 3) & ~3;
+    const int64_t tn = (int64_t)NP * NP;
+    FAIR_CHECK(tn < INT32_MAX, FAIR_ERR_ARG, "image too large for the window sweep");
+    if (A.t_side != N) {
+        const int64_t total = A.nnz + CSR_PAD;   // the spare entries too (column 0)
+        A.JAt.alloc((size_t)total);
+        hipLaunchKernelGGL(k_tile_cols, dim3(grid_for(total, 256)), dim3(256), 0, stream(), A.JA.p, A.JAt.p, total,
+                           A.n, N, NP);
+        launch_check("window tile columns");
+        A.t_side = N;
+    }
+    ArtWindowBufs &wb = A.win;
+    double *xt[2] = {nullptr, nullptr};
+    for (int c = 0; c < chains; ++c) {
+        if (wb.xt[c].n < (size_t)tn) wb.xt[c].alloc((size_t)tn);
+        xt[c] = wb.xt[c].p;
+        hipLaunchKernelGGL(k_x_to_tiles, dim3(grid_for(tn, 256)), dim3(256), 0, stream(), x[c], xt[c], N, NP);
+        launch_check("window tiles in");
+    }
+    for (int c = 0; c < chains; ++c)
+        art_window_prepare(A, A.JAt.p, (int32_t)tn, c, rows[c], nsteps[c], b, nrA, stream());
+    art_window_run(A, A.JAt.p, (int32_t)tn, chains, slots, xt, relax, lwrb, lb, upb, ub);
+    for (int c = 0; c < chains; ++c) {
+        hipLaunchKernelGGL(k_x_from_tiles, dim3(grid_for((int64_t)N * N, 256)), dim3(256), 0, stream(), xt[c], x[c],
+                           N, NP);
+        launch_check("window tiles out");
+    }
 }
 
 }  // namespace fair

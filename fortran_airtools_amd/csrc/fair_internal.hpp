@@ -186,6 +186,7 @@ struct ArtWindowBufs {
     DBuf<uint32_t> mask;
     DBuf<int32_t> slotof;
     DBuf<int32_t> err;
+    DBuf<double> xt[2];        // the chains' iterates in 4 x 4 pixel tiles (see Csr::JAt)
     int64_t nsteps[2] = {0, 0};   // steps each buffer set was prepared for
     int G = 0;
     int64_t n = 0;
@@ -220,6 +221,11 @@ struct Csr {
     HostArray<int32_t> hIA, hJA;                    // host copy of the pattern for level planning
                                                     // (kept up to 64 M entries)
     ArtWindowBufs win;
+    // The window sweep's column indices with the pixels of an N x N image renumbered in 4 x 4 tiles
+    // (a tile's 16 pixels are consecutive, so a ray's consecutive pixels share cache lines); built on
+    // first use for a square image, kept with the pattern it was made from
+    DBuf<int32_t> JAt;
+    int32_t t_side = 0;      // N of JAt (0: not built)
 };
 
 }  // namespace fair
@@ -293,11 +299,12 @@ void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const i
                        double *const x[2], const double *b, const double *nrA, double relax, int lwrb, double lb,
                        int upb, double ub);
 // Its two halves: the pre-pass into buffer set `slot` (0 or 1) on stream s, and the sweep over
-// prepared sets.
-void art_window_prepare(Csr &A, int slot, const int32_t *rows, int64_t nsteps, const double *b, const double *nrA,
-                        hipStream_t s);
-void art_window_run(Csr &A, int chains, const int slots[2], double *const x[2], double relax, int lwrb, double lb,
-                    int upb, double ub);
+// prepared sets.  cols / ncols: the column indices the sweep runs on (A.JA and A.n, or the tiled
+// numbering of launch_art_window) and the length of the iterates.
+void art_window_prepare(Csr &A, const int32_t *cols, int32_t ncols, int slot, const int32_t *rows, int64_t nsteps,
+                        const double *b, const double *nrA, hipStream_t s);
+void art_window_run(Csr &A, const int32_t *cols, int32_t ncols, int chains, const int slots[2], double *const x[2],
+                    double relax, int lwrb, double lb, int upb, double ub);
 
 // y[r] = sum_k vA[k] * x[JA[k]] over every row; empty rows give 0.
 void spmv(const Csr &A, const double *x, double *y);
