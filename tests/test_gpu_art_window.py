@@ -13,6 +13,7 @@ iterates must be equal, not merely close.  Covered:
 - rows above 512 entries (16 entries per lane, windows of 8 steps);
 - the N = 256 fan of the Kaczmarz benchmark;
 - the device-address entry point;
+- the 4 x 4-tile pixel numbering of square images against the image's own order;
 - the bounded wait: with a bound of zero polls the call fails with an error and the next call is
   unaffected.
 FAIR_DEBUG=1 makes the library print one line per window sweep, which shows that it ran."""
@@ -113,6 +114,31 @@ def test_twin_art_random_window_equals_sweep(monkeypatch, capfd, rule):
         monkeypatch, capfd, lambda: am.twin_art("randkaczmarz", rule, A, b, [3, 8], options=op, seed=3))
     assert used
     for a, c in zip(win, seq):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(c))
+
+
+def test_tiled_numbering_equals_image_order(monkeypatch):
+    """For a square image the window sweep runs on the pixels renumbered in 4 x 4 tiles; with
+    FAIR_ART_TILED=0 it runs on the image's own order.  Both give the same iterate, bit for bit: a
+    90 x 90 image (padded to 92 in the numbering), randomized Kaczmarz with box and relaxation, and
+    twin_art's two chains in one launch."""
+    _, _, am = fair()
+    monkeypatch.setenv("FAIR_ART_LEVELS", "0")
+    A = fan(90, 127, 64)
+    b = rhs(A, 13)
+    op = opts(am, relax=1.1, lb=0.0, ub=1.0)
+
+    def both(fn):
+        tiled = fn()
+        monkeypatch.setenv("FAIR_ART_TILED", "0")
+        plain = fn()
+        monkeypatch.delenv("FAIR_ART_TILED")
+        return tiled, plain
+
+    t, p = both(lambda: am.art("randkaczmarz", A, b, [1, 3], options=op, seed=2))
+    np.testing.assert_array_equal(t, p)
+    t, p = both(lambda: am.twin_art("randkaczmarz", "errorgauge", A, b, [4], options=op, seed=5))
+    for a, c in zip(t, p):
         np.testing.assert_array_equal(np.asarray(a), np.asarray(c))
 
 
