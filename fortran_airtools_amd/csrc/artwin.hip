@@ -617,10 +617,10 @@ void art_window_prepare(Csr &A, const int32_t *cols, int32_t ncols, int slot, co
 // The executor over `chains` chains (chain c: buffer set slots[c], prepared earlier in stream order),
 // iterate x[c] updated in place.  Waits for the sweep and reports a stop.
 void art_window_run(Csr &A, const int32_t *cols, int32_t ncols, int chains, const int slots[2], double *const x[2],
-                    double relax, int lwrb, double lb, int upb, double ub) {
+                    double relax, int lwrb, double lb, int upb, double ub, bool check) {
     const int R = win_R(A);
     ArtWindowBufs &wb = A.win;
-    if (wb.err.n == 0) wb.err.alloc(1);
+    FAIR_CHECK(wb.err.n >= 1, FAIR_ERR_INTERNAL, "window flag not cleared");
     WinChains ch{};
     for (int c = 0; c < 2; ++c) {
         const bool on = c < chains;
@@ -633,7 +633,6 @@ void art_window_run(Csr &A, const int32_t *cols, int32_t ncols, int chains, cons
         ch.cslot[c] = on ? wb.cslot[sl].p : nullptr;
         ch.cval[c] = on ? wb.cval[sl].p : nullptr;
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream(), (uint32_t *)wb.err.p, (int64_t)1);
     const int32_t spin = std::max(0, env_int("FAIR_ART_WINDOW_SPIN", 1 << 22));
     // the first spare entry after the matrix's entries (JA = 0, vA = 0, see alloc_entries)
     const int32_t pad = (int32_t)A.nnz;
@@ -647,6 +646,21 @@ void art_window_run(Csr &A, const int32_t *cols, int32_t ncols, int chains, cons
     if (env_int("FAIR_DEBUG", 0))
         fprintf(stderr, "[fair] window sweep: %lld + %lld steps, windows of %d, %d entries per lane\n",
                 (long long)ch.nsteps[0], (long long)ch.nsteps[1], R == 8 ? win_steps<8>() : win_steps<16>(), R);
+    if (check) art_window_check(A);
+}
+
+// The stop flag of the window sweeps since the last art_window_clear zeroed it, on stream order;
+// synchronises and fails the call when a step waited beyond its bound.
+void art_window_clear(Csr &A) {
+    ArtWindowBufs &wb = A.win;
+    if (wb.err.n == 0) wb.err.alloc(1);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream(), (uint32_t *)wb.err.p, (int64_t)1);
+    launch_check("window flag");
+}
+
+void art_window_check(Csr &A) {
+    ArtWindowBufs &wb = A.win;
+    if (wb.err.n == 0) return;
     int32_t herr = 0;
     FAIR_HIP(hipMemcpyAsync(&herr, wb.err.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
     FAIR_HIP(hipStreamSynchronize(stream()));
@@ -672,12 +686,13 @@ int32_t win_tile_side(const Csr &A) {
 // iterates are copied into it and back around the sweep.
 void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const int64_t nsteps[2],
                        double *const x[2], const double *b, const double *nrA, double relax, int lwrb, double lb,
-                       int upb, double ub) {
+                       int upb, double ub, bool deferred) {
     const int slots[2] = {0, 1};
+    if (!deferred) art_window_clear(A);
     const int32_t N = win_tile_side(A);
     if (N == 0) {
         for (int c = 0; c < chains; ++c) art_window_prepare(A, A.JA.p, A.n, c, rows[c], nsteps[c], b, nrA, stream());
-        art_window_run(A, A.JA.p, A.n, chains, slots, x, relax, lwrb, lb, upb, ub);
+        art_window_run(A, A.JA.p, A.n, chains, slots, x, relax, lwrb, lb, upb, ub, !deferred);
         return;
     }
     const int32_t NP = (N + 3) & ~3;
@@ -701,12 +716,13 @@ void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const i
     }
     for (int c = 0; c < chains; ++c)
         art_window_prepare(A, A.JAt.p, (int32_t)tn, c, rows[c], nsteps[c], b, nrA, stream());
-    art_window_run(A, A.JAt.p, (int32_t)tn, chains, slots, xt, relax, lwrb, lb, upb, ub);
+    art_window_run(A, A.JAt.p, (int32_t)tn, chains, slots, xt, relax, lwrb, lb, upb, ub, false);
     for (int c = 0; c < chains; ++c) {
         hipLaunchKernelGGL(k_x_from_tiles, dim3(grid_for((int64_t)N * N, 256)), dim3(256), 0, stream(), xt[c], x[c],
                            N, NP);
         launch_check("window tiles out");
     }
+    if (!deferred) art_window_check(A);
 }
 
 }  // namespace fair

@@ -294,17 +294,21 @@ inline int env_int(const char *name, int dflt) {
 
 // Kaczmarz window sweep (artwin.hip): the row steps of up to two chains (one-based rows on the
 // device), each iterate updated in place, bit-identical to the one-wavefront sweep.  Synchronous.
+// With deferred, the call neither clears nor reads the stop flag and does not synchronise: the
+// caller clears it before a run of sweeps (art_window_clear) and checks it after (art_window_check).
 bool art_window_usable(Csr &A, int64_t nsteps);   // FAIR_ART_WINDOW, rows up to 1024 entries
 void launch_art_window(Csr &A, int chains, const int32_t *const rows[2], const int64_t nsteps[2],
                        double *const x[2], const double *b, const double *nrA, double relax, int lwrb, double lb,
-                       int upb, double ub);
+                       int upb, double ub, bool deferred = false);
+void art_window_clear(Csr &A);
+void art_window_check(Csr &A);
 // Its two halves: the pre-pass into buffer set `slot` (0 or 1) on stream s, and the sweep over
 // prepared sets.  cols / ncols: the column indices the sweep runs on (A.JA and A.n, or the tiled
 // numbering of launch_art_window) and the length of the iterates.
 void art_window_prepare(Csr &A, const int32_t *cols, int32_t ncols, int slot, const int32_t *rows, int64_t nsteps,
                         const double *b, const double *nrA, hipStream_t s);
 void art_window_run(Csr &A, const int32_t *cols, int32_t ncols, int chains, const int slots[2], double *const x[2],
-                    double relax, int lwrb, double lb, int upb, double ub);
+                    double relax, int lwrb, double lb, int upb, double ub, bool check);
 
 // y[r] = sum_k vA[k] * x[JA[k]] over every row; empty rows give 0.
 void spmv(const Csr &A, const double *x, double *y);

@@ -388,14 +388,17 @@ void launch_art_sweep(Csr &A, unsigned chains, Args... args) {
 }
 
 // Row steps of up to two chains without a level schedule: the window sweep (artwin.hip) where it
-// applies, else the one-wavefront sweep; both give the same iterate bit for bit.
+// applies, else the one-wavefront sweep; both give the same iterate bit for bit.  deferred: the
+// window sweep's stop flag is cleared and checked by the caller (art_window_clear / _check), and
+// the call does not synchronise.
 void launch_art_chains(Csr &A, int chains, const int32_t *r0, int64_t n0, double *x0, const int32_t *r1, int64_t n1,
-                       double *x1, const double *b, const double *nrA, double relax, const Box &box) {
+                       double *x1, const double *b, const double *nrA, double relax, const Box &box,
+                       bool deferred = false) {
     if (art_window_usable(A, std::max(n0, n1))) {
         const int32_t *const rows[2] = {r0, r1};
         const int64_t ns[2] = {n0, n1};
         double *const xs[2] = {x0, x1};
-        launch_art_window(A, chains, rows, ns, xs, b, nrA, relax, box.lwrb, box.lb, box.upb, box.ub);
+        launch_art_window(A, chains, rows, ns, xs, b, nrA, relax, box.lwrb, box.lb, box.upb, box.ub, deferred);
         return;
     }
     launch_art_sweep(A, (unsigned)chains, SweepChains{{r0, r1}, {n0, n1}, {x0, x1}}, A.IA.p, A.JA.p, A.vA.p, b, nrA,
@@ -1471,6 +1474,40 @@ extern "C" int fair_twin_art(fair_csr *h, int32_t method, int32_t stoprule, cons
 
 namespace {
 
+// Two pinned host buffers and two device buffers for row sequences queued on the library's stream:
+// queue() waits until the upload last queued from the same host buffer has completed, copies the
+// sequence in, queues its upload and returns the device copy.  A device buffer is reused two
+// sequences later, after the sweep that read it (stream order).
+struct PinnedSeqs {
+    int32_t *host[2] = {nullptr, nullptr};
+    DBuf<int32_t> dev[2];
+    hipEvent_t done[2] = {nullptr, nullptr};
+    int next = 0;
+    void init(size_t count) {
+        for (int i = 0; i < 2; ++i) {
+            FAIR_HIP(hipHostMalloc((void **)&host[i], count * sizeof(int32_t), hipHostMallocDefault));
+            dev[i].alloc(count);
+            FAIR_HIP(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+        }
+    }
+    const int32_t *queue(const int32_t *seq, size_t count) {
+        const int i = next;
+        next ^= 1;
+        FAIR_HIP(hipEventSynchronize(done[i]));
+        std::copy(seq, seq + count, host[i]);
+        FAIR_HIP(hipMemcpyAsync(dev[i].p, host[i], count * sizeof(int32_t), hipMemcpyHostToDevice, stream()));
+        FAIR_HIP(hipEventRecord(done[i], stream()));
+        return dev[i].p;
+    }
+    ~PinnedSeqs() {
+        (void)hipStreamSynchronize(stream());
+        for (int i = 0; i < 2; ++i) {
+            if (done[i]) (void)hipEventDestroy(done[i]);
+            if (host[i]) (void)hipHostFree(host[i]);
+        }
+    }
+};
+
 // art, AIRMs.f90:59-369.  With xref (the reference's oracle), the iterate before each iteration is
 // kept and the run stops once the distance to xref grows: Xout(:, size(Kin)) = that iterate
 // (:322, 330-333).  norm2(xp - xo) of one iteration is norm2(x - xo) of the one before, so one
@@ -1509,6 +1546,15 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
     const int steps_per_iter = P.method == FAIR_SYMKACZMARZ ? 2 * P.nord : P.nord;
     std::vector<int32_t> seq(std::max(1, steps_per_iter));
     DBuf<int32_t> dseq(seq.size());
+    // Random orders in the window sweep run as a pipeline: the host draws the next iteration's
+    // sequence into one of two pinned buffers and queues its upload while the device still sweeps
+    // the one before; the sweeps' stop flag is checked once, after the last.
+    const bool piped = P.isrand && steps_per_iter > 0 && art_window_usable(h->A, steps_per_iter);
+    PinnedSeqs pipe;
+    if (piped) {
+        pipe.init((size_t)steps_per_iter);
+        art_window_clear(h->A);
+    }
     std::vector<int32_t> ord = P.ord;
     int64_t q = 0;
     if (P.isrand) {                                   // :297
@@ -1552,6 +1598,8 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
         if (rec < P.Kuse.size() && iter == P.Kuse[rec]) {
             dx.download(Xout + rec * (size_t)n, n);
             ++rec;
+        } else if (piped) {
+            return false;   // nothing to read back: the next iteration is queued behind this one
         }
         FAIR_HIP(hipStreamSynchronize(stream()));
         return false;
@@ -1565,6 +1613,10 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
         if (iter == 1 && !P.isrand) lev = art_levels(h->A, seq.data(), steps_per_iter);
         if (lev) {
             launch_art_levels(h->A, lev.get(), dx.p, nullptr, nullptr, db.p, d_nrA.p, relax, box);
+        } else if (piped) {
+            const int32_t *d_rows = pipe.queue(seq.data(), (size_t)steps_per_iter);
+            launch_art_chains(h->A, 1, d_rows, steps_per_iter, dx.p, nullptr, 0, nullptr, db.p, d_nrA.p, relax, box,
+                              true);
         } else if (steps_per_iter > 0) {
             dseq.upload(seq.data(), steps_per_iter);
             launch_art_chains(h->A, 1, dseq.p, steps_per_iter, dx.p, nullptr, 0, nullptr, db.p, d_nrA.p, relax,
@@ -1572,6 +1624,7 @@ void art_run(fair_csr *h, int32_t method, const double *b, int32_t nk, const int
         }
         if (after_sweep(iter)) break;
     }
+    if (piped) art_window_check(h->A);
     if (consumed) *consumed = q;
 }
 
