@@ -5,17 +5,16 @@
 // per-column scaling is applied in FP64 before rounding the result to FP32.
 //
 // Batched SIRT (k right-hand sides, e.g. k slices sharing one geometry): X is n x k and B, R are
-// m x k, row-major (the k values of one pixel / ray are contiguous).  k_spmm handles one matrix
-// row for SPMM_KC right-hand sides per wavefront: the 64 lanes are 64/SPMM_KC entry slots x
-// SPMM_KC right-hand sides, so every gathered matrix entry pulls SPMM_KC contiguous floats of X.
-// The chunk of right-hand sides a block works on is chosen from the block index so that the
-// blocks resident on one XCD (blocks are dealt round-robin to the 8 XCDs) share one chunk and
-// its slice of X stays in that XCD's L2.  The iteration loop can be captured once into a HIP
-// graph and replayed.
+// m x k, row-major (the k values of one pixel / ray are contiguous).  The SpMM kernels spread the
+// right-hand sides over the lanes, so every gathered matrix entry pulls a contiguous row segment
+// of X; the row order, the number of rows per wavefront (and with it the right-hand sides per
+// pass) and the tiled variant for A^T are tuned per matrix.  The iteration loop can be captured
+// once into a HIP graph and replayed.
 #include <algorithm>
 #include <cmath>
 #include <map>
 #include <tuple>
+#include <utility>
 
 #include "fair_internal.hpp"
 #include "spmv.hpp"
@@ -157,10 +156,17 @@ __global__ void __launch_bounds__(256) k_spmm_rows(const int4 *__restrict__ rows
         float acc[VEC];
 #pragma unroll
         for (int e = 0; e < VEC; ++e) acc[e] = 0.f;
+        // the next 64 entries are loaded (unconditionally, clamped to the row's last entry) while
+        // the current ones are gathered
+        const int last = max(ri.z - 1, ri.y);
+        int cn = JA[min(ri.y + lane, last)];
+        float vn = vA[min(ri.y + lane, last)];
         for (int base = ri.y; base < ri.z; base += WAVE) {
-            const int q = base + lane;
-            int cj = q < ri.z ? JA[q] : -1;
-            float vj = q < ri.z ? vA[q] : 0.f;
+            const bool in = base + lane < ri.z;
+            int cj = in ? cn : -1;
+            float vj = vn;
+            cn = JA[min(base + WAVE + lane, last)];
+            vn = vA[min(base + WAVE + lane, last)];
             if ((uint32_t)cj >= (uint32_t)ncols) {
                 cj = -1;
                 vj = 0.f;
@@ -185,6 +191,86 @@ __global__ void __launch_bounds__(256) k_spmm_rows(const int4 *__restrict__ rows
         }
         if (lane_in) epi.template apply<VEC>(ri.x, col, acc);
     }
+}
+
+// ---- batched: SpMM with several rows per wavefront ---------------------------------------------
+// k_spmm_groups gives each row a group of G = 16 or 32 lanes (64 / G rows per wavefront) and each
+// lane four right-hand sides, so one pass covers 4G of them; the grid's y index picks the chunk.
+// The hardware dispatches the y slices one after the other, so each slice is a full pass over the
+// rows (the matrix is read once per slice) whose rows in flight on an XCD touch 256 / 4G times
+// fewer cache lines of the panel than one pass over 256, while one gather instruction still moves
+// 64 / G row segments of 16G bytes (1 KB in all, as in k_spmm_rows).  Each 16-lane row of the
+// wavefront loads 16 consecutive entries of its group's matrix row and hands entry t to all its
+// lanes with the DPP row broadcast.  The panel is read through a buffer resource: an entry past its
+// row's end, or with a column outside [0, ncols), gets an offset beyond the resource's range,
+// which loads zeros, and weight 0.  Each output sums its row's entries in the stored order with the
+// same expression as k_spmm_rows, whose padding adds +0 the same way, so both give the same bits.
+constexpr uint32_t SPMM_OOB = 0x80000000u;   // a panel offset beyond every resource's range
+
+template <int N>
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v) {   // lane N of each 16-lane row, to the row
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + N, 0xF, 0xF, false);
+}
+
+// entries T0 .. T0 + SPMM_U - 1 of the 16 held by each 16-lane row
+template <int T0, int... U>
+__device__ __forceinline__ void spmm_span(__amdgpu_buffer_rsrc_t px, uint32_t oj, uint32_t wj, uint32_t offb,
+                                          float (&acc)[4], std::integer_sequence<int, U...>) {
+    const uint32_t o[SPMM_U] = {(row_bcast<T0 + U>(oj) + offb)...};
+    const uint32_t w[SPMM_U] = {row_bcast<T0 + U>(wj)...};
+    decltype(__builtin_amdgcn_raw_buffer_load_b128(px, 0u, 0, 0)) t[SPMM_U];
+#pragma unroll
+    for (int u = 0; u < SPMM_U; ++u) t[u] = __builtin_amdgcn_raw_buffer_load_b128(px, o[u], 0, 0);
+#pragma unroll
+    for (int u = 0; u < SPMM_U; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e] += __uint_as_float(w[u]) * __uint_as_float(t[u][e]);
+}
+
+template <int G, class Epi>
+__global__ void __launch_bounds__(256) k_spmm_groups(const int4 *__restrict__ rows, int32_t nrows, int32_t per_xcd,
+                                                     const int32_t *__restrict__ JA, const float *__restrict__ vA,
+                                                     int32_t ncols, const float *__restrict__ X, int32_t k, Epi epi) {
+    static_assert(G == 16 || G == 32, "16 or 32 lanes per row");
+    constexpr int RPW = WAVE / G;
+    const int b = blockIdx.x;
+    const int lb = (b % NXCD) * per_xcd + b / NXCD;
+    const int w0 = __builtin_amdgcn_readfirstlane((lb * SPMM_WAVES + (int)(threadIdx.x >> 6)) * RPW);
+    if (w0 >= nrows) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int g = lane / G, rl = lane & 15;
+    const int row = w0 + g;
+    const int4 ri = row < nrows ? rows[row] : make_int4(0, 0, 0, 0);
+    const int len = ri.z - ri.y;
+    int maxlen = 0;
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) maxlen = max(maxlen, __builtin_amdgcn_readlane(len, i * G));
+    const int c0 = (int)blockIdx.y * 4 * G;
+    const int col = c0 + 4 * (lane % G);
+    const uint32_t offb = 4u * (uint32_t)(col < k ? col : c0);   // lanes past k re-read the chunk's start
+    const uint32_t k4 = 4u * (uint32_t)k;
+    const __amdgpu_buffer_rsrc_t px =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(X), (short)0, (int)((uint32_t)ncols * k4), 0x00020000);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    // The next 16 entries are loaded while the current ones are gathered.  The loads are
+    // unconditional (an entry past the row's end re-reads the row's last one, or entry 0 of the
+    // matrix for an empty group) so that waiting for the current ones never waits for them.
+    const int last = ri.y + max(len - 1, 0);
+    int cn = JA[min(ri.y + rl, last)];
+    float vn = vA[min(ri.y + rl, last)];
+    for (int base = 0; base < maxlen; base += 16) {
+        const bool in = base + rl < len;
+        const int cj = in ? cn : -1;
+        const float vj = vn;
+        cn = JA[min(ri.y + base + 16 + rl, last)];
+        vn = vA[min(ri.y + base + 16 + rl, last)];
+        const bool ok = (uint32_t)cj < (uint32_t)ncols;
+        const uint32_t oj = ok ? (uint32_t)cj * k4 : SPMM_OOB;
+        const uint32_t wj = ok ? __float_as_uint(vj) : 0u;
+        spmm_span<0>(px, oj, wj, offb, acc, std::make_integer_sequence<int, SPMM_U>{});
+        if (maxlen - base > SPMM_U) spmm_span<SPMM_U>(px, oj, wj, offb, acc, std::make_integer_sequence<int, SPMM_U>{});
+    }
+    if (row < nrows && col < k) epi.template apply<4>(ri.x, col, acc);
 }
 
 template <class Epi>
@@ -272,9 +358,23 @@ constexpr int PIX_TILES[SPMM_ORDERS][3] = {{0, 0, 0},   {8, 8, 0},   {16, 16, 0}
                                            {16, 32, 0}, {32, 16, 0}, {16, 8, 0},  {8, 16, 0},
                                            {32, 8, 0},  {64, 16, 0}, {16, 16, 1}, {64, 64, 0}};
 
+// Code c + SPMM_GROUP_CODE * s (s = 1, 2) runs k_spmm_groups with SPMM_GROUP_LANES[s] lanes per row
+// in row order c.
+constexpr int SPMM_GROUP_CODE = 1000;
+constexpr int SPMM_GROUPINGS = 3;
+constexpr int SPMM_GROUP_LANES[SPMM_GROUPINGS] = {WAVE, 32, 16};
+
+// Whether grouping s (> 0) applies to k right-hand sides over ncols panel rows: k a multiple of 4
+// and wider than one pass, the panel addressable by the buffer resource with room for SPMM_OOB.
+bool groups_apply(int s, int32_t k, int32_t ncols) {
+    if (s <= 0 || s >= SPMM_GROUPINGS || k % 4 != 0 || k <= 4 * SPMM_GROUP_LANES[s]) return false;
+    return (uint64_t)ncols * (uint64_t)k * 4u + 4u * (uint64_t)k <= SPMM_OOB;
+}
+
 // The row descriptors of M in order `code` (built once per code and cached in M); rays says
 // whether M's rows are rays (A of a ray-traced matrix) or pixels (its transpose).
 const int4 *spmm_order(const fair_csr *h, Csr &M, bool rays, int code) {
+    code %= SPMM_GROUP_CODE;
     const bool geo = h->g_n > 0 && h->g_p > 0 && h->g_nth > 0;
     if (code <= 0 || code >= SPMM_ORDERS || !geo || M.n_rows_nz <= 0) return (const int4 *)M.rows_nz.p;
     if (M.spmm_code == code && M.spmm_rows.p) return (const int4 *)M.spmm_rows.p;
@@ -298,6 +398,7 @@ const int4 *spmm_order(const fair_csr *h, Csr &M, bool rays, int code) {
     std::sort(key.begin(), key.end());
     std::vector<int4> o((size_t)nr);
     for (int64_t s = 0; s < nr; ++s) o[s] = d[key[s].second];
+    if (h->graphs_used) drop_graphs(h);   // captured launches refer to the order being replaced
     M.spmm_rows.alloc(4 * (size_t)nr);
     FAIR_HIP(hipMemcpyAsync(M.spmm_rows.p, o.data(), sizeof(int4) * nr, hipMemcpyHostToDevice, stream()));
     FAIR_HIP(hipStreamSynchronize(stream()));
@@ -318,6 +419,7 @@ constexpr int STILE_CHUNK = 64;
 constexpr int STILE_SHAPES = 5;
 constexpr int STILE[STILE_SHAPES][2] = {{4, 4}, {8, 2}, {2, 8}, {16, 1}, {1, 16}};
 constexpr int STILE_CODE0 = 100;   // order codes STILE_CODE0 + i select the tiled kernel, shape i
+bool is_tiled(int code) { return code >= STILE_CODE0 && code < STILE_CODE0 + STILE_SHAPES; }
 
 template <class Epi>
 __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ binfo, int32_t nblocks, int32_t per_xcd,
@@ -435,6 +537,7 @@ bool build_stiles(const fair_csr *h, Csr &M, int shape) {
             binfo.push_back(make_int4(rb, nr, (int32_t)uni.size(), (int32_t)u.size()));
             uni.insert(uni.end(), u.begin(), u.end());
         }
+    if (h->graphs_used) drop_graphs(h);   // captured launches refer to the layout being replaced
     auto up = [](DBuf<int32_t> &d, const void *src, size_t ints) {
         d.alloc(ints > 0 ? ints : 1);
         if (ints) FAIR_HIP(hipMemcpyAsync(d.p, src, ints * sizeof(int32_t), hipMemcpyHostToDevice, stream()));
@@ -476,6 +579,16 @@ void spmm_launch_v(const int4 *rows, int32_t nrows, const Csr &M, const float *X
                        (int32_t)per_xcd, M.JA.p, M.vA32.p, M.n, X, k, zero_row(), epi);
 }
 
+template <int G, class Epi>
+void spmm_launch_g(const int4 *rows, int32_t nrows, const Csr &M, const float *X, int32_t k, Epi epi, hipStream_t s) {
+    const int64_t waves = (nrows + WAVE / G - 1) / (WAVE / G);
+    const int64_t blocks = (waves + SPMM_WAVES - 1) / SPMM_WAVES;
+    const int64_t per_xcd = (blocks + NXCD - 1) / NXCD;
+    const int64_t slices = (k + 4 * G - 1) / (4 * G);
+    hipLaunchKernelGGL((k_spmm_groups<G, Epi>), dim3((unsigned)(per_xcd * NXCD), (unsigned)slices), dim3(256), 0, s,
+                       rows, nrows, (int32_t)per_xcd, M.JA.p, M.vA32.p, M.n, X, k, epi);
+}
+
 // Everything the products of a batched run build lazily (row orders, tiled layouts), made ahead.
 void spmm_prepare(const fair_csr *h, int32_t k) {
     Csr &A = const_cast<fair_csr *>(h)->A, &T = const_cast<fair_csr *>(h)->T;
@@ -493,8 +606,12 @@ void spmm_launch(const fair_csr *h, Csr &M, bool rays, const float *X, int32_t k
         spmm_launch_tiles(M, X, k, epi, s);
         return;
     }
-    const int4 *rows = spmm_order(h, M, rays, code >= STILE_CODE0 ? 0 : code);
-    if (k % 4 == 0) spmm_launch_v<4>(rows, M.n_rows_nz, M, X, k, epi, s);
+    const int order = code % SPMM_GROUP_CODE, grouping = code / SPMM_GROUP_CODE;
+    const int4 *rows = spmm_order(h, M, rays, is_tiled(order) ? 0 : order);
+    if (groups_apply(grouping, k, M.n)) {
+        if (SPMM_GROUP_LANES[grouping] == 32) spmm_launch_g<32>(rows, M.n_rows_nz, M, X, k, epi, s);
+        else spmm_launch_g<16>(rows, M.n_rows_nz, M, X, k, epi, s);
+    } else if (k % 4 == 0) spmm_launch_v<4>(rows, M.n_rows_nz, M, X, k, epi, s);
     else spmm_launch_v<1>(rows, M.n_rows_nz, M, X, k, epi, s);
 }
 
@@ -717,9 +834,13 @@ extern "C" int fair_batch_relerr_dev(const float *d_X, const double *d_Xref, int
 }
 
 // Autotuner of the batched SpMM: times every row order (sinogram tiles for A, image tiles for
-// A^T; only the stored order for a matrix without geometry) on this matrix with k right-hand
-// sides with HIP events (two launches each, after a warm-up) and keeps the fastest per
-// direction in the handle.
+// A^T; only the stored order for a matrix without geometry), each with one row per wavefront and,
+// where k allows, with two and four rows per wavefront (passes of 128 and 64 right-hand sides), on
+// this matrix with k right-hand sides with HIP events (two launches each, after a warm-up); then
+// times the pairs of the SPMM_FINALISTS fastest per product as whole iterations and keeps the
+// fastest pair in the handle.
+constexpr int SPMM_FINALISTS = 3;
+constexpr int SPMM_PAIR_ITS = 4;   // even: the iterate ends in the buffer it started in
 extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
     return guard([&] {
         check_handle(h);
@@ -737,18 +858,19 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
         FAIR_HIP(hipEventCreate(&e0));
         FAIR_HIP(hipEventCreate(&e1));
         const bool dbg = getenv("FAIR_DEBUG") && atoi(getenv("FAIR_DEBUG")) != 0;
+        std::vector<std::pair<float, int>> timed[2];   // (ms, code) per product
         for (int dir = 0; dir < 2; ++dir) {
             Csr &M = dir == 0 ? A : T;
             const bool rays = dir == 0 ? a_rays : !a_rays;
             const float *src = dir == 0 ? X.p : R.p;
-            float best = 1e30f;
-            int best_code = 0;
             std::vector<int> codes;
-            for (int c = 0; c < (geo ? SPMM_ORDERS : 1); ++c) codes.push_back(c);
+            for (int gs = 0; gs < SPMM_GROUPINGS; ++gs)
+                if (gs == 0 || groups_apply(gs, k, M.n))
+                    for (int c = 0; c < (geo ? SPMM_ORDERS : 1); ++c) codes.push_back(c + SPMM_GROUP_CODE * gs);
             for (int c = 0; c < STILE_SHAPES; ++c)
                 if (geo && !rays && k % 4 == 0 && k <= 4 * WAVE) codes.push_back(STILE_CODE0 + c);
             for (int code : codes) {
-                if (code >= STILE_CODE0 && !stiles_apply(h, M, rays, code, k)) continue;
+                if (is_tiled(code) && !stiles_apply(h, M, rays, code, k)) continue;
                 auto run = [&] { spmm_launch(h, M, rays, src, k, code, EpiStoreB{Y.p, k}, stream()); };
                 run();
                 FAIR_HIP(hipEventRecord(e0, stream()));
@@ -758,23 +880,55 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                 FAIR_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
                 FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (dbg && code >= STILE_CODE0) {
+                if (dbg && is_tiled(code)) {
                     fprintf(stderr, "[fair] spmm %s tiled %d (%dx%d pixels, LDS-staged union): %.1f us\n",
                             dir == 0 ? "A" : "A^T", code, STILE[code - STILE_CODE0][0], STILE[code - STILE_CODE0][1],
                             ms * 500.f);
                 } else if (dbg) {
-                    const int *tl = rays ? RAY_TILES[code] : PIX_TILES[code];
-                    fprintf(stderr, "[fair] spmm %s order %d (%dx%d%s): %.1f us\n", dir == 0 ? "A" : "A^T", code,
-                            tl[0] == ALL ? -1 : tl[0], tl[1], tl[2] ? ", band major" : "", ms * 500.f);
+                    const int oc = code % SPMM_GROUP_CODE, gs = code / SPMM_GROUP_CODE;
+                    const int *tl = rays ? RAY_TILES[oc] : PIX_TILES[oc];
+                    fprintf(stderr, "[fair] spmm %s order %d (%dx%d%s), %d lanes per row: %.1f us\n",
+                            dir == 0 ? "A" : "A^T", code, tl[0] == ALL ? -1 : tl[0], tl[1],
+                            tl[2] ? ", band major" : "", SPMM_GROUP_LANES[gs], ms * 500.f);
                 }
+                timed[dir].emplace_back(ms, code);
+            }
+            std::stable_sort(timed[dir].begin(), timed[dir].end());
+            if (timed[dir].size() > (size_t)SPMM_FINALISTS) timed[dir].resize(SPMM_FINALISTS);
+        }
+        // The finalists' pairs run as whole SART iterations (residual and update epilogues, both
+        // matrices and four panels streaming through the caches as in a run), which the single
+        // products above do not show: a product read in several passes re-reads its matrix from
+        // HBM there.  The fastest pair is kept.
+        DBuf<float> Bz((size_t)A.m * k);
+        FAIR_HIP(hipMemsetAsync(Bz.p, 0, Bz.bytes(), stream()));
+        float best = 1e30f;
+        int best_pair[2] = {timed[0].empty() ? 0 : timed[0][0].second, timed[1].empty() ? 0 : timed[1][0].second};
+        for (const auto &ta : timed[0])
+            for (const auto &tt : timed[1]) {
+                h->spmm_tile[0] = ta.second;
+                h->spmm_tile[1] = tt.second;
+                spmm_prepare(h, k);
+                batch_iterations(h, FAIR_SART, Bz.p, k, 2, X.p, R.p, Y.p, nullptr, stream());
+                FAIR_HIP(hipEventRecord(e0, stream()));
+                batch_iterations(h, FAIR_SART, Bz.p, k, SPMM_PAIR_ITS, X.p, R.p, Y.p, nullptr, stream());
+                FAIR_HIP(hipEventRecord(e1, stream()));
+                FAIR_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (dbg)
+                    fprintf(stderr, "[fair] spmm orders %d, %d: %.1f us per iteration\n", ta.second, tt.second,
+                            ms * 1000.f / SPMM_PAIR_ITS);
                 if (ms < best) {
                     best = ms;
-                    best_code = code;
+                    best_pair[0] = ta.second;
+                    best_pair[1] = tt.second;
                 }
             }
-            h->spmm_tile[dir] = best_code;
-            if (!stiles_apply(h, M, rays, best_code, k)) spmm_order(h, M, rays, best_code);
-        }
+        h->spmm_tile[0] = best_pair[0];
+        h->spmm_tile[1] = best_pair[1];
+        spmm_prepare(h, k);
+        FAIR_HIP(hipStreamSynchronize(stream()));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         launch_check("autotune batch");

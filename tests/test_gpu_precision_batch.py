@@ -32,7 +32,8 @@ def test_fp32_sirt_within_1e5_of_fp64(method):
 def batch_inputs(A, k):
     from fortran_airtools_amd import configs as cf
     gen = [cf.shepplogan, cf.bubbles, cf.smooth, cf.threephases, cf.grains]
-    X = [gen[j % len(gen)](64) for j in range(k)]
+    N = int(round(np.sqrt(A.n)))
+    X = [gen[j % len(gen)](N) for j in range(k)]
     B = np.stack([cf.add_noise(A.matvecmul(x), 0.01, 100 + j) for j, x in enumerate(X)], 1)
     return B
 
@@ -96,6 +97,33 @@ def test_batch_graph_survives_workspace_and_matrix_changes():
     assert g.tobytes() != first.tobytes()
 
 
+def test_batch_graph_after_row_order_changes(monkeypatch):
+    """A changed row order or tiled layout replaces the matrix's cached copy, which captured runs
+    refer to: graph runs over orders 3,6 -> 2,100 (tiled A^T) -> 2002,10 (four rows per wavefront)
+    -> 3,6 again each equal the eager run bit for bit."""
+    import torch
+    from fortran_airtools_amd import airms
+    A = c1()
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    k = 256
+    B = torch.from_numpy(batch_inputs(A, k).astype(np.float32)).cuda()
+    X = torch.zeros((A.n, k), dtype=torch.float32, device="cuda")
+
+    def run(graph):
+        X.zero_()
+        torch.cuda.synchronize()
+        airms.sirt_batch_dev("sart", A, B.data_ptr(), k, 6, X.data_ptr(), options=o, use_graph=graph)
+        torch.cuda.synchronize()
+        return X.cpu().numpy()
+
+    monkeypatch.setenv("FAIR_SPMM_ORDERS", "3,6")
+    eager = run(False)
+    for codes in ("3,6", "2,100", "2002,10", "3,6"):
+        monkeypatch.setenv("FAIR_SPMM_ORDERS", codes)
+        assert run(True).tobytes() == eager.tobytes(), codes
+        assert run(True).tobytes() == eager.tobytes(), codes
+
+
 @pytest.mark.parametrize("tile_code", [100, 101, 102, 103, 104])
 def test_tiled_transpose_product_equals_row_kernel(monkeypatch, tile_code):
     """The LDS-staged tiled kernel of A^T*R (pixels of an image tile, their rays' union staged in
@@ -116,3 +144,39 @@ def test_tiled_transpose_product_equals_row_kernel(monkeypatch, tile_code):
             torch.cuda.synchronize()
             out[codes] = X.cpu().numpy()
         assert out["3,6"].tobytes() == out[f"3,{tile_code}"].tobytes(), k
+
+
+def parallel_pi():
+    """Parallel beam, N = 96, 90 angles up to exactly pi: the tracer emits 77 column numbers above
+    N^2 at the last angle (a reference quirk kept in the matrix), which the products read as zero."""
+    from fortran_airtools_amd import system_generation as sg
+    return sg.paralleltomo(sg.Data(n=96, p=137, nth=90, theta=np.deg2rad(np.linspace(0.0, 180.0, 90))))
+
+
+@pytest.mark.parametrize("geom", ["fan_c1", "parallel_n96_pi"])
+@pytest.mark.parametrize("grouping", [1, 2])
+def test_grouped_rows_equal_one_row_per_wavefront(monkeypatch, geom, grouping):
+    """Both products with two rows per wavefront in passes of 128 right-hand sides (grouping 1) or
+    four rows in passes of 64 (grouping 2) give the same bits as one row per wavefront over all of
+    them: each output sums the same entries in the same order.  k = 96 (grouping 1 does not apply
+    and runs one row per wavefront), 132 (a last pass of 4) and 256; row lengths differ inside a
+    wavefront, and the parallel-beam matrix has columns outside the image."""
+    import torch
+    from fortran_airtools_amd import airms
+    A = c1() if geom == "fan_c1" else parallel_pi()
+    if geom == "parallel_n96_pi":
+        assert int((np.asarray(A.JA) >= A.n).sum()) > 0
+    o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
+    for k in (96, 132, 256):
+        B = torch.from_numpy(batch_inputs(A, k).astype(np.float32)).cuda()
+        out = {}
+        group_codes = f"{1000 * grouping + 3},{1000 * grouping + 6}"
+        for codes in ("3,6", group_codes):
+            monkeypatch.setenv("FAIR_SPMM_ORDERS", codes)
+            X = torch.zeros((A.n, k), dtype=torch.float32, device="cuda")
+            torch.cuda.synchronize()
+            airms.sirt_batch_dev("sart", A, B.data_ptr(), k, 12, X.data_ptr(), options=o)
+            torch.cuda.synchronize()
+            out[codes] = X.cpu().numpy()
+        assert np.isfinite(out["3,6"]).all() and np.abs(out["3,6"]).max() > 0, k
+        assert out["3,6"].tobytes() == out[group_codes].tobytes(), k
