@@ -359,10 +359,25 @@ constexpr int PIX_TILES[SPMM_ORDERS][3] = {{0, 0, 0},   {8, 8, 0},   {16, 16, 0}
                                            {32, 8, 0},  {64, 16, 0}, {16, 16, 1}, {64, 64, 0}};
 
 // Code c + SPMM_GROUP_CODE * s (s = 1, 2) runs k_spmm_groups with SPMM_GROUP_LANES[s] lanes per row
-// in row order c.
+// in row order c; + SPMM_CAP_CODE * b (b = 2..6) lets at most b of the row kernels' workgroups be
+// resident on a compute unit (each reserves 1 / b of its LDS), which bounds the rows in flight on
+// an XCD and with them the panel lines they share its L2 with.
 constexpr int SPMM_GROUP_CODE = 1000;
 constexpr int SPMM_GROUPINGS = 3;
 constexpr int SPMM_GROUP_LANES[SPMM_GROUPINGS] = {WAVE, 32, 16};
+constexpr int SPMM_CAP_CODE = 10000;
+constexpr int SPMM_CAPS[] = {6, 5, 4, 3, 2};
+
+int code_order(int code) { return code % SPMM_GROUP_CODE; }
+int code_grouping(int code) { return (code / SPMM_GROUP_CODE) % (SPMM_CAP_CODE / SPMM_GROUP_CODE); }
+
+// LDS bytes a row-kernel workgroup reserves so that at most `cap` fit on a compute unit (0: none)
+size_t cap_lds(int code) {
+    const int cap = code / SPMM_CAP_CODE;
+    if (cap <= 0) return 0;
+    const size_t b = ((size_t)lds_per_cu() / (size_t)cap) & ~(size_t)1023;
+    return std::min(b, (size_t)lds_per_block());
+}
 
 // Whether grouping s (> 0) applies to k right-hand sides over ncols panel rows: k a multiple of 4
 // and wider than one pass, the panel addressable by the buffer resource with room for SPMM_OOB.
@@ -571,21 +586,22 @@ void spmm_launch_tiles(const Csr &M, const float *X, int32_t k, Epi epi, hipStre
 }
 
 template <int VEC, class Epi>
-void spmm_launch_v(const int4 *rows, int32_t nrows, const Csr &M, const float *X, int32_t k, Epi epi,
+void spmm_launch_v(const int4 *rows, int32_t nrows, const Csr &M, const float *X, int32_t k, Epi epi, size_t lds,
                    hipStream_t s) {
     const int64_t blocks = (nrows + SPMM_WAVES - 1) / SPMM_WAVES;
     const int64_t per_xcd = (blocks + NXCD - 1) / NXCD;
-    hipLaunchKernelGGL((k_spmm_rows<VEC, Epi>), dim3((unsigned)(per_xcd * NXCD)), dim3(256), 0, s, rows, nrows,
+    hipLaunchKernelGGL((k_spmm_rows<VEC, Epi>), dim3((unsigned)(per_xcd * NXCD)), dim3(256), lds, s, rows, nrows,
                        (int32_t)per_xcd, M.JA.p, M.vA32.p, M.n, X, k, zero_row(), epi);
 }
 
 template <int G, class Epi>
-void spmm_launch_g(const int4 *rows, int32_t nrows, const Csr &M, const float *X, int32_t k, Epi epi, hipStream_t s) {
+void spmm_launch_g(const int4 *rows, int32_t nrows, const Csr &M, const float *X, int32_t k, Epi epi, size_t lds,
+                   hipStream_t s) {
     const int64_t waves = (nrows + WAVE / G - 1) / (WAVE / G);
     const int64_t blocks = (waves + SPMM_WAVES - 1) / SPMM_WAVES;
     const int64_t per_xcd = (blocks + NXCD - 1) / NXCD;
     const int64_t slices = (k + 4 * G - 1) / (4 * G);
-    hipLaunchKernelGGL((k_spmm_groups<G, Epi>), dim3((unsigned)(per_xcd * NXCD), (unsigned)slices), dim3(256), 0, s,
+    hipLaunchKernelGGL((k_spmm_groups<G, Epi>), dim3((unsigned)(per_xcd * NXCD), (unsigned)slices), dim3(256), lds, s,
                        rows, nrows, (int32_t)per_xcd, M.JA.p, M.vA32.p, M.n, X, k, epi);
 }
 
@@ -606,13 +622,14 @@ void spmm_launch(const fair_csr *h, Csr &M, bool rays, const float *X, int32_t k
         spmm_launch_tiles(M, X, k, epi, s);
         return;
     }
-    const int order = code % SPMM_GROUP_CODE, grouping = code / SPMM_GROUP_CODE;
+    const int order = code_order(code), grouping = code_grouping(code);
+    const size_t lds = cap_lds(code);
     const int4 *rows = spmm_order(h, M, rays, is_tiled(order) ? 0 : order);
     if (groups_apply(grouping, k, M.n)) {
-        if (SPMM_GROUP_LANES[grouping] == 32) spmm_launch_g<32>(rows, M.n_rows_nz, M, X, k, epi, s);
-        else spmm_launch_g<16>(rows, M.n_rows_nz, M, X, k, epi, s);
-    } else if (k % 4 == 0) spmm_launch_v<4>(rows, M.n_rows_nz, M, X, k, epi, s);
-    else spmm_launch_v<1>(rows, M.n_rows_nz, M, X, k, epi, s);
+        if (SPMM_GROUP_LANES[grouping] == 32) spmm_launch_g<32>(rows, M.n_rows_nz, M, X, k, epi, lds, s);
+        else spmm_launch_g<16>(rows, M.n_rows_nz, M, X, k, epi, lds, s);
+    } else if (k % 4 == 0) spmm_launch_v<4>(rows, M.n_rows_nz, M, X, k, epi, lds, s);
+    else spmm_launch_v<1>(rows, M.n_rows_nz, M, X, k, epi, lds, s);
 }
 
 // Cached HIP graphs of whole batched runs, keyed by the run's arguments.
@@ -837,8 +854,9 @@ extern "C" int fair_batch_relerr_dev(const float *d_X, const double *d_Xref, int
 // A^T; only the stored order for a matrix without geometry), each with one row per wavefront and,
 // where k allows, with two and four rows per wavefront (passes of 128 and 64 right-hand sides), on
 // this matrix with k right-hand sides with HIP events (two launches each, after a warm-up); then
-// times the pairs of the SPMM_FINALISTS fastest per product as whole iterations and keeps the
-// fastest pair in the handle.
+// times the pairs of the SPMM_FINALISTS fastest per product as whole iterations, then the
+// residency caps of the fastest pair's products one after the other, and keeps the fastest
+// choice in the handle.
 constexpr int SPMM_FINALISTS = 3;
 constexpr int SPMM_PAIR_ITS = 4;   // even: the iterate ends in the buffer it started in
 extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
@@ -885,7 +903,7 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                             dir == 0 ? "A" : "A^T", code, STILE[code - STILE_CODE0][0], STILE[code - STILE_CODE0][1],
                             ms * 500.f);
                 } else if (dbg) {
-                    const int oc = code % SPMM_GROUP_CODE, gs = code / SPMM_GROUP_CODE;
+                    const int oc = code_order(code), gs = code_grouping(code);
                     const int *tl = rays ? RAY_TILES[oc] : PIX_TILES[oc];
                     fprintf(stderr, "[fair] spmm %s order %d (%dx%d%s), %d lanes per row: %.1f us\n",
                             dir == 0 ? "A" : "A^T", code, tl[0] == ALL ? -1 : tl[0], tl[1],
@@ -925,9 +943,33 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                     best_pair[1] = tt.second;
                 }
             }
+        // Then each product's residency cap, one product after the other, on whole iterations.
         h->spmm_tile[0] = best_pair[0];
         h->spmm_tile[1] = best_pair[1];
         spmm_prepare(h, k);
+        for (int dir = 0; dir < 2; ++dir) {
+            const int base = h->spmm_tile[dir];
+            if (is_tiled(base)) continue;
+            int best_code = base;
+            for (int cap : SPMM_CAPS) {
+                h->spmm_tile[dir] = base + SPMM_CAP_CODE * cap;
+                batch_iterations(h, FAIR_SART, Bz.p, k, 2, X.p, R.p, Y.p, nullptr, stream());
+                FAIR_HIP(hipEventRecord(e0, stream()));
+                batch_iterations(h, FAIR_SART, Bz.p, k, SPMM_PAIR_ITS, X.p, R.p, Y.p, nullptr, stream());
+                FAIR_HIP(hipEventRecord(e1, stream()));
+                FAIR_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (dbg)
+                    fprintf(stderr, "[fair] spmm orders %d, %d: %.1f us per iteration\n", h->spmm_tile[0],
+                            h->spmm_tile[1], ms * 1000.f / SPMM_PAIR_ITS);
+                if (ms < best) {
+                    best = ms;
+                    best_code = h->spmm_tile[dir];
+                }
+            }
+            h->spmm_tile[dir] = best_code;
+        }
         FAIR_HIP(hipStreamSynchronize(stream()));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

@@ -315,6 +315,7 @@ void spmv(const Csr &A, const double *x, double *y);
 void fill(double *y, int64_t n, double v);
 int64_t exclusive_scan_i32(const int32_t *d_counts, int64_t m, int32_t *d_out);
 int lds_per_block();
+int lds_per_cu();   // LDS bytes of one compute unit
 void row_norms(const Csr &A, int p, double *out);  // (sum |a|^p)^(1/p), pow-based
 
 // Matrix-free projector support in raytrace.hip: the rays whose entries must come from the

@@ -29,6 +29,7 @@ namespace {
 thread_local std::string g_last_error;
 hipStream_t g_stream = nullptr;
 int g_lds_per_block = 65536;
+int g_lds_per_cu = 65536;
 }  // namespace
 
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -59,6 +60,7 @@ void host_free_huge(void *p) { free(p); }
 
 hipStream_t stream() { return g_stream; }
 int lds_per_block() { return g_lds_per_block; }
+int lds_per_cu() { return g_lds_per_cu; }
 
 void launch_check(const char *what) {
     hipError_t e = hipGetLastError();
@@ -439,6 +441,8 @@ extern "C" int fair_init(int device) {
         hipDeviceProp_t prop;
         FAIR_HIP(hipGetDeviceProperties(&prop, device));
         g_lds_per_block = (int)prop.sharedMemPerBlock;
+        g_lds_per_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (int)prop.maxSharedMemoryPerMultiProcessor
+                                                                 : g_lds_per_block;
         sirt_scratch_addr();   // the no-allocation SIRT's array on this device
     });
 }

@@ -168,8 +168,9 @@ int fair_sirt_batch_dev(fair_csr *A, int32_t method, const float *d_B, int32_t k
  * image tiles for A^T*R; codes 0-11, 0 = stored order) and, for A^T*R, the LDS-staged tiled kernel
  * (codes 100-104: 4x4, 8x2, 2x8, 16x1, 1x16 pixel tiles) on this matrix with k right-hand sides,
  * the row orders also with two rows per wavefront in passes of 128 right-hand sides (code + 1000) and
- * four rows in passes of 64 (code + 2000), and keeps the fastest in the handle (order_out[2] receives
- * the codes, may be NULL).  All choices give the same results. */
+ * four rows in passes of 64 (code + 2000), then at most b = 2..6 workgroups resident per compute unit
+ * for the row kernels (code + 10000 b), and keeps the fastest in the handle (order_out[2] receives the
+ * codes, may be NULL).  All choices give the same results. */
 int fair_autotune_batch(fair_csr *A, int32_t k, int32_t *order_out);
 /* Host-array form of fair_sirt_batch_dev (B: m x k, X0 may be NULL, Xout: n x k). */
 int fair_sirt_batch(fair_csr *A, int32_t method, const float *B, int32_t k, int32_t maxits,

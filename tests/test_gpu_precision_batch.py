@@ -100,7 +100,7 @@ def test_batch_graph_survives_workspace_and_matrix_changes():
 def test_batch_graph_after_row_order_changes(monkeypatch):
     """A changed row order or tiled layout replaces the matrix's cached copy, which captured runs
     refer to: graph runs over orders 3,6 -> 2,100 (tiled A^T) -> 2002,10 (four rows per wavefront)
-    -> 3,6 again each equal the eager run bit for bit."""
+    -> 32002,40010 (with residency caps) -> 3,6 again each equal the eager run bit for bit."""
     import torch
     from fortran_airtools_amd import airms
     A = c1()
@@ -118,7 +118,7 @@ def test_batch_graph_after_row_order_changes(monkeypatch):
 
     monkeypatch.setenv("FAIR_SPMM_ORDERS", "3,6")
     eager = run(False)
-    for codes in ("3,6", "2,100", "2002,10", "3,6"):
+    for codes in ("3,6", "2,100", "2002,10", "32002,40010", "3,6"):
         monkeypatch.setenv("FAIR_SPMM_ORDERS", codes)
         assert run(True).tobytes() == eager.tobytes(), codes
         assert run(True).tobytes() == eager.tobytes(), codes
@@ -157,8 +157,9 @@ def parallel_pi():
 @pytest.mark.parametrize("grouping", [1, 2])
 def test_grouped_rows_equal_one_row_per_wavefront(monkeypatch, geom, grouping):
     """Both products with two rows per wavefront in passes of 128 right-hand sides (grouping 1) or
-    four rows in passes of 64 (grouping 2) give the same bits as one row per wavefront over all of
-    them: each output sums the same entries in the same order.  k = 96 (grouping 1 does not apply
+    four rows in passes of 64 (grouping 2), also with a cap on the workgroups resident per compute
+    unit, give the same bits as one row per wavefront over all of them: each output sums the same
+    entries in the same order.  k = 96 (grouping 1 does not apply
     and runs one row per wavefront), 132 (a last pass of 4) and 256; row lengths differ inside a
     wavefront, and the parallel-beam matrix has columns outside the image."""
     import torch
@@ -171,7 +172,8 @@ def test_grouped_rows_equal_one_row_per_wavefront(monkeypatch, geom, grouping):
         B = torch.from_numpy(batch_inputs(A, k).astype(np.float32)).cuda()
         out = {}
         group_codes = f"{1000 * grouping + 3},{1000 * grouping + 6}"
-        for codes in ("3,6", group_codes):
+        capped = f"{30000 + 1000 * grouping + 3},{50006}"   # at most 3 / 5 workgroups per CU
+        for codes in ("3,6", group_codes, capped):
             monkeypatch.setenv("FAIR_SPMM_ORDERS", codes)
             X = torch.zeros((A.n, k), dtype=torch.float32, device="cuda")
             torch.cuda.synchronize()
@@ -180,3 +182,4 @@ def test_grouped_rows_equal_one_row_per_wavefront(monkeypatch, geom, grouping):
             out[codes] = X.cpu().numpy()
         assert np.isfinite(out["3,6"]).all() and np.abs(out["3,6"]).max() > 0, k
         assert out["3,6"].tobytes() == out[group_codes].tobytes(), k
+        assert out["3,6"].tobytes() == out[capped].tobytes(), k
