@@ -422,86 +422,143 @@ const int4 *spmm_order(const fair_csr *h, Csr &M, bool rays, int code) {
 }
 
 // ---- batched: tiled SpMM for rows that share their columns (pixels of A^T) ------------------------
-// A block of up to 16 wavefronts takes the rows of one tile (pixels of a small image tile, which are
-// crossed by largely the same rays).  The union of the tile's columns, sorted, is staged through LDS
-// in chunks of STILE_CHUNK panel rows; each wavefront walks its own row's entries (ascending, so
-// every chunk's entries of a row are contiguous) and reads the gathered rows from LDS.  A union
-// column is fetched from L2 once per tile instead of once per entry.  Each lane sums its row's
-// entries in the stored order with the same expression as k_spmm_rows, so both kernels give the
-// same bits.  One pass over k <= 256 right-hand sides (k a multiple of 4).
+// A block of 16 wavefronts takes the rows of one tile (pixels of a small image tile, which are
+// crossed by largely the same rays), one, two or four rows per wavefront (16-, 32- or 64-pixel
+// tiles).  The union of the tile's columns, sorted, is staged through LDS in chunks of CH panel
+// rows, double-buffered: chunk c + 1 is loaded into registers while chunk c is summed and written to the
+// other buffer at the end of the step, and one barrier per chunk orders both.  Each row's entries
+// of a chunk are a prefix of a 64-entry window (LDS slot and value, one per lane; the slots
+// ascend), found with ballot + popcount.  Gathered rows are read from LDS (1 KB per wavefront read
+// at k = 256) instead of through the texture data path, which bounds the row kernels.  Each lane
+// sums its row's entries in the stored order with the same expression as k_spmm_rows, so both
+// kernels give the same bits.  One pass over k <= 256 right-hand sides (k a multiple of 4).
 constexpr int STILE_ROWS = 16;
-constexpr int STILE_CHUNK = 64;
-constexpr int STILE_SHAPES = 5;
-constexpr int STILE[STILE_SHAPES][2] = {{4, 4}, {8, 2}, {2, 8}, {16, 1}, {1, 16}};
-constexpr int STILE_CODE0 = 100;   // order codes STILE_CODE0 + i select the tiled kernel, shape i
-bool is_tiled(int code) { return code >= STILE_CODE0 && code < STILE_CODE0 + STILE_SHAPES; }
+constexpr int STILE_SHAPES = 9;   // 16, 32 or 64 pixels: one, two or four per wavefront
+constexpr int STILE[STILE_SHAPES][2] = {{4, 4}, {8, 2}, {2, 8}, {16, 1}, {1, 16}, {8, 4}, {4, 8}, {8, 8}, {16, 4}};
+constexpr int STILE_CHUNKS = 2;
+constexpr int STILE_CH[STILE_CHUNKS] = {64, 32};
+// order codes STILE_CODE0 + 10 * c + i select the tiled kernel, shape i, chunk STILE_CH[c]
+constexpr int STILE_CODE0 = 100;
+bool is_tiled(int code) {
+    return code >= STILE_CODE0 && code < STILE_CODE0 + 10 * STILE_CHUNKS && code % 10 < STILE_SHAPES;
+}
+int tiled_shape(int code) { return (code - STILE_CODE0) % 10; }
+int tiled_chunk(int code) { return STILE_CH[(code - STILE_CODE0) / 10]; }
+size_t tiled_lds(int code, int32_t k) { return 2 * (size_t)tiled_chunk(code) * (size_t)k * sizeof(float); }
 
-template <class Epi>
+template <int CH, int RPW, class Epi>
 __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ binfo, int32_t nblocks, int32_t per_xcd,
                                                      const int4 *__restrict__ rows, const int32_t *__restrict__ uni,
                                                      const uint16_t *__restrict__ slot, const float *__restrict__ vA,
                                                      const float *__restrict__ X, int32_t k, Epi epi) {
-    extern __shared__ float4 stage[];   // STILE_CHUNK rows of k / 4 float4
+    extern __shared__ float4 stage[];   // two buffers of CH rows of k / 4 float4
+    constexpr int PER_WAVE = CH / STILE_ROWS;
+    constexpr int PAST = 1 << 20;       // slot of an entry past the row's end
     const int b = blockIdx.x;
     const int lb = (b % NXCD) * per_xcd + b / NXCD;
     if (lb >= nblocks) return;          // the whole block leaves together
-    const int4 bi = binfo[lb];
-    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & (WAVE - 1));
-    const int kv = k >> 2;
-    const bool has = w < bi.y;
-    const int4 ri = has ? rows[bi.x + w] : make_int4(0, 0, 0, 0);
-    int q = ri.y;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    // the next chunk's panel rows are loaded into registers while the current chunk is used
-    constexpr int PER_WAVE = STILE_CHUNK / STILE_ROWS;
-    float4 pre[PER_WAVE];
-    auto fetch = [&](int c0) {
-        const int cn = min(STILE_CHUNK, bi.w - c0);
+    const int4 bi = binfo[lb];          // first row, rows, union start, union size
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & (WAVE - 1));
+    const int kv = k >> 2, kl = min(lane, kv - 1);
+    // this wavefront's rows of the tile: w, w + 16, ...
+    bool has[RPW];
+    int4 ri[RPW];
+    int last[RPW];   // entry loads are clamped here (entry 0 without a row)
 #pragma unroll
-        for (int i = 0; i < PER_WAVE; ++i) {
-            const int u = w + STILE_ROWS * i;
-            pre[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (u < cn && lane < kv) {
-                const int col = uni[bi.z + c0 + u];
-                pre[i] = reinterpret_cast<const float4 *>(X + (int64_t)col * k)[lane];
-            }
-        }
-    };
-    fetch(0);
-    for (int c0 = 0; c0 < bi.w; c0 += STILE_CHUNK) {
-        const int cn = min(STILE_CHUNK, bi.w - c0);
-#pragma unroll
-        for (int i = 0; i < PER_WAVE; ++i) {
-            const int u = w + STILE_ROWS * i;
-            if (u < cn && lane < kv) stage[u * kv + lane] = pre[i];
-        }
-        __syncthreads();
-        if (c0 + STILE_CHUNK < bi.w) fetch(c0 + STILE_CHUNK);
-        if (has) {
-            for (;;) {
-                const int qq = q + lane;
-                const int sl = qq < ri.z ? (int)slot[qq] : 1 << 20;
-                const float vv = qq < ri.z ? vA[qq] : 0.f;
-                const uint64_t inmask = __ballot(sl < c0 + cn);   // a prefix: the slots ascend
-                const int nin = __popcll(inmask);
-                for (int t = 0; t < nin; ++t) {
-                    const int su = __builtin_amdgcn_readlane(sl, t) - c0;
-                    const float wv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vv), t));
-                    if (lane < kv) {
-                        const float4 xv = stage[su * kv + lane];
-                        acc[0] += wv * xv.x;
-                        acc[1] += wv * xv.y;
-                        acc[2] += wv * xv.z;
-                        acc[3] += wv * xv.w;
-                    }
-                }
-                q += nin;
-                if (nin < WAVE) break;   // the rest of the row lies in later chunks
-            }
-        }
-        __syncthreads();
+    for (int j = 0; j < RPW; ++j) {
+        has[j] = w + STILE_ROWS * j < bi.y;
+        ri[j] = has[j] ? rows[bi.x + w + STILE_ROWS * j] : make_int4(0, 0, 0, 0);
+        last[j] = max(ri[j].z - 1, ri[j].y);
     }
-    if (has && lane < kv) epi.template apply<4>(ri.x, lane * 4, acc);
+    // the union rows of chunk c0 (all union numbers first, then all gathers; rows past the union
+    // re-read its last one, lanes past k / 4 the row's last float4, neither is stored)
+    FV<4> pre[PER_WAVE];   // plain floats: the rows stay in registers
+    int col[PER_WAVE];
+#pragma unroll
+    for (int i = 0; i < PER_WAVE; ++i) col[i] = uni[bi.z + min(w + STILE_ROWS * i, bi.w - 1)];
+#pragma unroll
+    for (int i = 0; i < PER_WAVE; ++i) pre[i] = FV<4>::load(X + (int64_t)col[i] * k + 4 * kl);
+#pragma unroll
+    for (int i = 0; i < PER_WAVE; ++i)
+        if (w + STILE_ROWS * i < min(CH, bi.w) && lane < kv)
+            stage[(w + STILE_ROWS * i) * kv + lane] = make_float4(pre[i].v[0], pre[i].v[1], pre[i].v[2], pre[i].v[3]);
+    // Each row's entries from q on, one per lane.  A chunk holds at most CH <= 64 of a row's entries
+    // (the layout is built only where every row's slots strictly ascend), so one window per chunk
+    // holds them all; the window for chunk c + 1 is loaded at the end of step c, after the gathers
+    // of chunk c + 1 were issued and before they are waited for.  Every step issues the same loads
+    // in the same order (the last step's gathers re-read the union's last row, a missing row loads
+    // entry 0; neither is used), so that the waits between them are exact.
+    int q[RPW], sl[RPW];
+    float va[RPW];
+    float acc[RPW][4];
+#pragma unroll
+    for (int j = 0; j < RPW; ++j) {
+        q[j] = ri[j].y;
+        sl[j] = (int)slot[min(q[j] + lane, last[j])];
+        va[j] = vA[min(q[j] + lane, last[j])];
+        if (q[j] + lane >= ri[j].z) sl[j] = PAST;
+        acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0.f;
+    }
+    const int nch = (bi.w + CH - 1) / CH;
+    for (int c = 0; c < nch; ++c) {
+        const int c0 = c * CH, cn = min(CH, bi.w - c0);
+        const float4 *buf = stage + (c & 1) * CH * kv;
+        // chunk c is in its buffer; every wavefront is done with chunk c - 1, whose buffer takes
+        // chunk c + 1 at the end of this step
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) col[i] = uni[bi.z + min(c0 + CH + w + STILE_ROWS * i, bi.w - 1)];
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) pre[i] = FV<4>::load(X + (int64_t)col[i] * k + 4 * kl);
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            const int nin = has[j] ? __popcll(__ballot(sl[j] < c0 + cn)) : 0;   // a prefix: slots ascend
+            int t = 0;
+            for (; t + 4 <= nin; t += 4) {
+                float4 xv[4];
+                float wv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int su = __builtin_amdgcn_readlane(sl[j], t + u) - c0;
+                    wv[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(va[j]), t + u));
+                    xv[u] = buf[su * kv + kl];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    acc[j][0] += wv[u] * xv[u].x;
+                    acc[j][1] += wv[u] * xv[u].y;
+                    acc[j][2] += wv[u] * xv[u].z;
+                    acc[j][3] += wv[u] * xv[u].w;
+                }
+            }
+            for (; t < nin; ++t) {
+                const int su = __builtin_amdgcn_readlane(sl[j], t) - c0;
+                const float wv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(va[j]), t));
+                const float4 xv = buf[su * kv + kl];
+                acc[j][0] += wv * xv.x;
+                acc[j][1] += wv * xv.y;
+                acc[j][2] += wv * xv.z;
+                acc[j][3] += wv * xv.w;
+            }
+            q[j] += nin;   // the rest of the row lies in later chunks
+        }
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            sl[j] = (int)slot[min(q[j] + lane, last[j])];
+            va[j] = vA[min(q[j] + lane, last[j])];
+            if (q[j] + lane >= ri[j].z) sl[j] = PAST;
+        }
+        float4 *nb = stage + ((c + 1) & 1) * CH * kv;
+        const int nn = min(CH, bi.w - c0 - CH);
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i)
+            if (w + STILE_ROWS * i < nn && lane < kv)
+                nb[(w + STILE_ROWS * i) * kv + lane] = make_float4(pre[i].v[0], pre[i].v[1], pre[i].v[2], pre[i].v[3]);
+    }
+#pragma unroll
+    for (int j = 0; j < RPW; ++j)
+        if (has[j] && lane < kv) epi.template apply<4>(ri[j].x, lane * 4, acc[j]);
 }
 
 // The tiled layout of M (rows = pixels, j = x * N + y, columns = rays) for tile shape i: built on
@@ -548,7 +605,12 @@ bool build_stiles(const fair_csr *h, Csr &M, int shape) {
             std::sort(u.begin(), u.end());
             for (size_t i = 0; i < u.size(); ++i) pos[u[i]] = (int32_t)i;
             for (int32_t r = rb; r < rb + nr; ++r)
-                for (int32_t e = rows[r].y; e < rows[r].z; ++e) slot[e] = (uint16_t)pos[JA[e]];
+                for (int32_t e = rows[r].y; e < rows[r].z; ++e) {
+                    slot[e] = (uint16_t)pos[JA[e]];
+                    // the kernel takes a chunk's entries of a row as a prefix of at most one per
+                    // union row: every row's columns must strictly ascend
+                    if (e > rows[r].y && slot[e] <= slot[e - 1]) return false;
+                }
             binfo.push_back(make_int4(rb, nr, (int32_t)uni.size(), (int32_t)u.size()));
             uni.insert(uni.end(), u.begin(), u.end());
         }
@@ -570,17 +632,21 @@ bool build_stiles(const fair_csr *h, Csr &M, int shape) {
 // Whether order code `code` selects the tiled kernel for M with k right-hand sides (and its layout
 // is built); pixel rows only.
 bool stiles_apply(const fair_csr *h, Csr &M, bool rays, int code, int32_t k) {
-    if (rays || code < STILE_CODE0 || code >= STILE_CODE0 + STILE_SHAPES || k % 4 != 0 || k > 4 * WAVE) return false;
-    if ((size_t)STILE_CHUNK * (size_t)k * sizeof(float) > (size_t)lds_per_block()) return false;
-    return build_stiles(h, M, code - STILE_CODE0);
+    if (rays || !is_tiled(code) || k % 4 != 0 || k > 4 * WAVE) return false;
+    if (tiled_lds(code, k) > (size_t)lds_per_block()) return false;
+    return build_stiles(h, M, tiled_shape(code));
 }
 
 template <class Epi>
-void spmm_launch_tiles(const Csr &M, const float *X, int32_t k, Epi epi, hipStream_t s) {
+void spmm_launch_tiles(const Csr &M, const float *X, int32_t k, int code, Epi epi, hipStream_t s) {
     const int64_t blocks = M.stile.nblocks;
     const int64_t per_xcd = (blocks + NXCD - 1) / NXCD;
-    const size_t lds = (size_t)STILE_CHUNK * (size_t)k * sizeof(float);
-    hipLaunchKernelGGL((k_spmm_tiles<Epi>), dim3((unsigned)(per_xcd * NXCD)), dim3(64 * STILE_ROWS), lds, s,
+    const size_t lds = tiled_lds(code, k);
+    const int rpw = STILE[tiled_shape(code)][0] * STILE[tiled_shape(code)][1] / STILE_ROWS;
+    auto kern = tiled_chunk(code) == 64
+                    ? (rpw == 4 ? k_spmm_tiles<64, 4, Epi> : rpw == 2 ? k_spmm_tiles<64, 2, Epi> : k_spmm_tiles<64, 1, Epi>)
+                    : (rpw == 4 ? k_spmm_tiles<32, 4, Epi> : rpw == 2 ? k_spmm_tiles<32, 2, Epi> : k_spmm_tiles<32, 1, Epi>);
+    hipLaunchKernelGGL(kern, dim3((unsigned)(per_xcd * NXCD)), dim3(64 * STILE_ROWS), lds, s,
                        (const int4 *)M.stile.binfo.p, (int32_t)blocks, (int32_t)per_xcd, (const int4 *)M.stile.rows.p,
                        M.stile.uni.p, M.stile.slot.p, M.vA32.p, X, k, epi);
 }
@@ -619,7 +685,7 @@ void spmm_launch(const fair_csr *h, Csr &M, bool rays, const float *X, int32_t k
                  hipStream_t s) {
     if (M.n_rows_nz <= 0 || k <= 0) return;
     if (stiles_apply(h, M, rays, code, k)) {
-        spmm_launch_tiles(M, X, k, epi, s);
+        spmm_launch_tiles(M, X, k, code, epi, s);
         return;
     }
     const int order = code_order(code), grouping = code_grouping(code);
@@ -885,8 +951,9 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
             for (int gs = 0; gs < SPMM_GROUPINGS; ++gs)
                 if (gs == 0 || groups_apply(gs, k, M.n))
                     for (int c = 0; c < (geo ? SPMM_ORDERS : 1); ++c) codes.push_back(c + SPMM_GROUP_CODE * gs);
-            for (int c = 0; c < STILE_SHAPES; ++c)
-                if (geo && !rays && k % 4 == 0 && k <= 4 * WAVE) codes.push_back(STILE_CODE0 + c);
+            for (int ci = 0; ci < STILE_CHUNKS; ++ci)
+                for (int c = 0; c < STILE_SHAPES; ++c)
+                    if (geo && !rays && k % 4 == 0 && k <= 4 * WAVE) codes.push_back(STILE_CODE0 + 10 * ci + c);
             for (int code : codes) {
                 if (is_tiled(code) && !stiles_apply(h, M, rays, code, k)) continue;
                 auto run = [&] { spmm_launch(h, M, rays, src, k, code, EpiStoreB{Y.p, k}, stream()); };
@@ -899,9 +966,9 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                 float ms = 0.f;
                 FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
                 if (dbg && is_tiled(code)) {
-                    fprintf(stderr, "[fair] spmm %s tiled %d (%dx%d pixels, LDS-staged union): %.1f us\n",
-                            dir == 0 ? "A" : "A^T", code, STILE[code - STILE_CODE0][0], STILE[code - STILE_CODE0][1],
-                            ms * 500.f);
+                    fprintf(stderr, "[fair] spmm %s tiled %d (%dx%d pixels, LDS-staged union, %d-row chunks): %.1f us\n",
+                            dir == 0 ? "A" : "A^T", code, STILE[tiled_shape(code)][0], STILE[tiled_shape(code)][1],
+                            tiled_chunk(code), ms * 500.f);
                 } else if (dbg) {
                     const int oc = code_order(code), gs = code_grouping(code);
                     const int *tl = rays ? RAY_TILES[oc] : PIX_TILES[oc];

@@ -124,11 +124,15 @@ def test_batch_graph_after_row_order_changes(monkeypatch):
         assert run(True).tobytes() == eager.tobytes(), codes
 
 
-@pytest.mark.parametrize("tile_code", [100, 101, 102, 103, 104])
+@pytest.mark.parametrize("tile_code", [100, 101, 102, 103, 104, 105, 106, 107, 108,
+                                       110, 111, 112, 113, 114, 115, 116, 117, 118])
 def test_tiled_transpose_product_equals_row_kernel(monkeypatch, tile_code):
     """The LDS-staged tiled kernel of A^T*R (pixels of an image tile, their rays' union staged in
-    chunks) gives the same bits as the wavefront-per-row kernel: every pixel's sum runs over its
-    entries in the stored order with the same operations.  k = 32 and k = 256 right-hand sides."""
+    double-buffered chunks of 64 (codes 100-108) or 32 (110-118) rows; 16-pixel tiles one pixel per
+    wavefront, 8 x 4 and 4 x 8 tiles (codes x05, x06) two, 8 x 8 and 16 x 4 (x07, x08) four) gives
+    the same bits as the
+    wavefront-per-row kernel: every pixel's sum runs over its entries in the stored order with the
+    same operations.  k = 32 and k = 256 right-hand sides."""
     import torch
     from fortran_airtools_amd import airms
     A = c1()
