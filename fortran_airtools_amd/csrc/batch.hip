@@ -12,6 +12,7 @@
 // once into a HIP graph and replayed.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <map>
 #include <tuple>
 #include <utility>
@@ -433,8 +434,10 @@ const int4 *spmm_order(const fair_csr *h, Csr &M, bool rays, int code) {
 // sums its row's entries in the stored order with the same expression as k_spmm_rows, so both
 // kernels give the same bits.  One pass over k <= 256 right-hand sides (k a multiple of 4).
 constexpr int STILE_ROWS = 16;
-constexpr int STILE_SHAPES = 9;   // 16, 32 or 64 pixels: one, two or four per wavefront
+constexpr int STILE_SHAPES = 9;   // 16, 32 or 64 pixels (x by y): one, two or four per wavefront
 constexpr int STILE[STILE_SHAPES][2] = {{4, 4}, {8, 2}, {2, 8}, {16, 1}, {1, 16}, {8, 4}, {4, 8}, {8, 8}, {16, 4}};
+constexpr int SRAY_SHAPES = 6;    // 16, 32 or 64 rays (angles by detectors)
+constexpr int SRAY[SRAY_SHAPES][2] = {{8, 8}, {16, 4}, {4, 16}, {8, 4}, {4, 8}, {4, 4}};
 constexpr int STILE_CHUNKS = 2;
 constexpr int STILE_CH[STILE_CHUNKS] = {64, 32};
 // order codes STILE_CODE0 + 10 * c + i select the tiled kernel, shape i, chunk STILE_CH[c]
@@ -561,58 +564,106 @@ __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ bi
         if (has[j] && lane < kv) epi.template apply<4>(ri[j].x, lane * 4, acc[j]);
 }
 
-// The tiled layout of M (rows = pixels, j = x * N + y, columns = rays) for tile shape i: built on
-// the host from M's pattern once per shape and cached in M; false when it does not apply (a tile's
-// union exceeds 65,535 columns, or a column lies outside the matrix).
-bool build_stiles(const fair_csr *h, Csr &M, int shape) {
+// The tiled layout of M for tile shape i: rows are pixels (j = x * N + y, tiles of STILE[i] pixels)
+// or rays (r = angle * p + detector, tiles of SRAY[i] angles x detectors).  Built on the host from
+// M's pattern once per shape and cached in M; false when it does not apply (a column outside the
+// matrix, or a row that repeats a column).
+bool build_stiles(const fair_csr *h, Csr &M, bool rays, int shape) {
     if (M.stile.code == shape) return M.stile.nblocks > 0;
     M.stile.code = shape;
     M.stile.nblocks = 0;
-    const int64_t N = h->g_n;
-    if (N <= 0 || (int64_t)M.m != N * N || M.nnz <= 0) return false;
+    // rows r = u * V + v: pixels j = x * N + y of an N x N image, or rays r = angle * p + detector
+    const int64_t U = rays ? h->g_nth : h->g_n, V = rays ? h->g_p : h->g_n;
+    if (U <= 0 || V <= 0 || (int64_t)M.m != U * V || M.nnz <= 0) return false;
+    if (shape < 0 || shape >= (rays ? SRAY_SHAPES : STILE_SHAPES)) return false;
     std::vector<int32_t> IA((size_t)M.m + 1), JA((size_t)M.nnz);
     M.IA.download(IA.data(), (size_t)M.m + 1);
     M.JA.download(JA.data(), (size_t)M.nnz);
     FAIR_HIP(hipStreamSynchronize(stream()));
-    const int TX = STILE[shape][0], TY = STILE[shape][1];
+    const int TU = rays ? SRAY[shape][0] : STILE[shape][0], TV = rays ? SRAY[shape][1] : STILE[shape][1];
     std::vector<int4> binfo, rows;
     std::vector<int32_t> uni;
     std::vector<uint16_t> slot((size_t)M.nnz);
     std::vector<int32_t> mark((size_t)std::max(M.n, 1), -1), pos((size_t)std::max(M.n, 1), 0);
-    std::vector<int32_t> u;
-    int32_t tile = 0;
-    for (int64_t x0 = 0; x0 < N; x0 += TX)
-        for (int64_t y0 = 0; y0 < N; y0 += TY, ++tile) {
-            const int32_t rb = (int32_t)rows.size();
-            u.clear();
-            for (int64_t x = x0; x < std::min(N, x0 + TX); ++x)
-                for (int64_t y = y0; y < std::min(N, y0 + TY); ++y) {
-                    const int64_t j = x * N + y;
-                    if (IA[j + 1] == IA[j]) continue;
-                    rows.push_back(make_int4((int32_t)j, IA[j], IA[j + 1], 0));
-                    for (int32_t e = IA[j]; e < IA[j + 1]; ++e) {
-                        const int32_t c = JA[e];
-                        if ((uint32_t)c >= (uint32_t)M.n) return false;
-                        if (mark[c] != tile) {
-                            mark[c] = tile;
-                            u.push_back(c);
-                        }
-                    }
+    std::vector<int32_t> u, indeg, first, succ, order, fill;
+    int32_t stamp = 0;
+    // One block of the layout from rows rw[0, n) (non-empty, in tile order): the union of their
+    // columns in an order every row follows (sorted for pixels, whose rays ascend; Kahn's algorithm
+    // on the precedence of consecutive entries for rays).  Where there is no such order, or the
+    // union is too large, the rows are split in two halves (the earlier angles first) and placed
+    // separately.  0: the layout does not apply (a column outside the matrix, or a single row that
+    // repeats a column).
+    std::function<int(const int64_t *, size_t)> place = [&](const int64_t *rw, size_t n) -> int {
+        ++stamp;
+        u.clear();
+        for (size_t i = 0; i < n; ++i)
+            for (int32_t e = IA[rw[i]]; e < IA[rw[i] + 1]; ++e) {
+                const int32_t c = JA[e];
+                if ((uint32_t)c >= (uint32_t)M.n) return 0;
+                if (mark[c] != stamp) {
+                    mark[c] = stamp;
+                    pos[c] = (int32_t)u.size();
+                    u.push_back(c);
                 }
-            const int32_t nr = (int32_t)rows.size() - rb;
-            if (nr == 0) continue;
-            if (u.size() > 65535) return false;
+            }
+        bool ok = u.size() <= 65535;
+        if (ok && !rays) {
             std::sort(u.begin(), u.end());
-            for (size_t i = 0; i < u.size(); ++i) pos[u[i]] = (int32_t)i;
-            for (int32_t r = rb; r < rb + nr; ++r)
-                for (int32_t e = rows[r].y; e < rows[r].z; ++e) {
-                    slot[e] = (uint16_t)pos[JA[e]];
-                    // the kernel takes a chunk's entries of a row as a prefix of at most one per
-                    // union row: every row's columns must strictly ascend
-                    if (e > rows[r].y && slot[e] <= slot[e - 1]) return false;
+        } else if (ok) {
+            const size_t nu = u.size();
+            indeg.assign(nu, 0);
+            first.assign(nu + 1, 0);
+            for (size_t i = 0; i < n; ++i)
+                for (int32_t e = IA[rw[i]] + 1; e < IA[rw[i] + 1]; ++e) ++first[(size_t)pos[JA[e - 1]] + 1];
+            for (size_t i = 0; i < nu; ++i) first[i + 1] += first[i];
+            succ.assign((size_t)first[nu], 0);
+            fill.assign(first.begin(), first.end() - 1);
+            for (size_t i = 0; i < n; ++i)
+                for (int32_t e = IA[rw[i]] + 1; e < IA[rw[i] + 1]; ++e) {
+                    const int32_t to = pos[JA[e]];
+                    succ[(size_t)fill[(size_t)pos[JA[e - 1]]]++] = to;
+                    ++indeg[(size_t)to];
                 }
-            binfo.push_back(make_int4(rb, nr, (int32_t)uni.size(), (int32_t)u.size()));
-            uni.insert(uni.end(), u.begin(), u.end());
+            order.clear();
+            for (size_t i = 0; i < nu; ++i)
+                if (indeg[i] == 0) order.push_back((int32_t)i);
+            for (size_t q = 0; q < order.size(); ++q)
+                for (int32_t f = first[(size_t)order[q]]; f < first[(size_t)order[q] + 1]; ++f)
+                    if (--indeg[(size_t)succ[(size_t)f]] == 0) order.push_back(succ[(size_t)f]);
+            ok = order.size() == nu;
+            if (ok) {
+                std::vector<int32_t> ou(nu);
+                for (size_t i = 0; i < nu; ++i) ou[i] = u[(size_t)order[i]];
+                u.swap(ou);
+            }
+        }
+        if (ok) {
+            for (size_t i = 0; i < u.size(); ++i) pos[u[i]] = (int32_t)i;
+            // the kernel takes a chunk's entries of a row as a prefix of at most one per union
+            // row: every row's slots must strictly ascend
+            for (size_t i = 0; i < n && ok; ++i)
+                for (int32_t e = IA[rw[i]]; e < IA[rw[i] + 1]; ++e) {
+                    slot[e] = (uint16_t)pos[JA[e]];
+                    if (e > IA[rw[i]] && slot[e] <= slot[e - 1]) ok = false;
+                }
+        }
+        if (!ok) {
+            if (n == 1) return 0;
+            return place(rw, n / 2) && place(rw + n / 2, n - n / 2);
+        }
+        binfo.push_back(make_int4((int32_t)rows.size(), (int32_t)n, (int32_t)uni.size(), (int32_t)u.size()));
+        for (size_t i = 0; i < n; ++i) rows.push_back(make_int4((int32_t)rw[i], IA[rw[i]], IA[rw[i] + 1], 0));
+        uni.insert(uni.end(), u.begin(), u.end());
+        return 1;
+    };
+    std::vector<int64_t> rw;
+    for (int64_t u0 = 0; u0 < U; u0 += TU)
+        for (int64_t v0 = 0; v0 < V; v0 += TV) {
+            rw.clear();
+            for (int64_t a = u0; a < std::min(U, u0 + TU); ++a)
+                for (int64_t b = v0; b < std::min(V, v0 + TV); ++b)
+                    if (IA[a * V + b + 1] > IA[a * V + b]) rw.push_back(a * V + b);
+            if (!rw.empty() && !place(rw.data(), rw.size())) return false;
         }
     if (h->graphs_used) drop_graphs(h);   // captured launches refer to the layout being replaced
     auto up = [](DBuf<int32_t> &d, const void *src, size_t ints) {
@@ -630,19 +681,20 @@ bool build_stiles(const fair_csr *h, Csr &M, int shape) {
 }
 
 // Whether order code `code` selects the tiled kernel for M with k right-hand sides (and its layout
-// is built); pixel rows only.
+// is built).
 bool stiles_apply(const fair_csr *h, Csr &M, bool rays, int code, int32_t k) {
-    if (rays || !is_tiled(code) || k % 4 != 0 || k > 4 * WAVE) return false;
+    if (!is_tiled(code) || k % 4 != 0 || k > 4 * WAVE) return false;
     if (tiled_lds(code, k) > (size_t)lds_per_block()) return false;
-    return build_stiles(h, M, tiled_shape(code));
+    return build_stiles(h, M, rays, tiled_shape(code));
 }
 
 template <class Epi>
-void spmm_launch_tiles(const Csr &M, const float *X, int32_t k, int code, Epi epi, hipStream_t s) {
+void spmm_launch_tiles(const Csr &M, bool rays, const float *X, int32_t k, int code, Epi epi, hipStream_t s) {
     const int64_t blocks = M.stile.nblocks;
     const int64_t per_xcd = (blocks + NXCD - 1) / NXCD;
     const size_t lds = tiled_lds(code, k);
-    const int rpw = STILE[tiled_shape(code)][0] * STILE[tiled_shape(code)][1] / STILE_ROWS;
+    const int *shp = rays ? SRAY[tiled_shape(code)] : STILE[tiled_shape(code)];
+    const int rpw = shp[0] * shp[1] / STILE_ROWS;
     auto kern = tiled_chunk(code) == 64
                     ? (rpw == 4 ? k_spmm_tiles<64, 4, Epi> : rpw == 2 ? k_spmm_tiles<64, 2, Epi> : k_spmm_tiles<64, 1, Epi>)
                     : (rpw == 4 ? k_spmm_tiles<32, 4, Epi> : rpw == 2 ? k_spmm_tiles<32, 2, Epi> : k_spmm_tiles<32, 1, Epi>);
@@ -685,7 +737,7 @@ void spmm_launch(const fair_csr *h, Csr &M, bool rays, const float *X, int32_t k
                  hipStream_t s) {
     if (M.n_rows_nz <= 0 || k <= 0) return;
     if (stiles_apply(h, M, rays, code, k)) {
-        spmm_launch_tiles(M, X, k, code, epi, s);
+        spmm_launch_tiles(M, rays, X, k, code, epi, s);
         return;
     }
     const int order = code_order(code), grouping = code_grouping(code);
@@ -918,11 +970,12 @@ extern "C" int fair_batch_relerr_dev(const float *d_X, const double *d_Xref, int
 
 // Autotuner of the batched SpMM: times every row order (sinogram tiles for A, image tiles for
 // A^T; only the stored order for a matrix without geometry), each with one row per wavefront and,
-// where k allows, with two and four rows per wavefront (passes of 128 and 64 right-hand sides), on
+// where k allows, with two and four rows per wavefront (passes of 128 and 64 right-hand sides),
+// and the LDS-staged tiled kernel's tile shapes and chunk sizes for both products, on
 // this matrix with k right-hand sides with HIP events (two launches each, after a warm-up); then
 // times the pairs of the SPMM_FINALISTS fastest per product as whole iterations, then the
-// residency caps of the fastest pair's products one after the other, and keeps the fastest
-// choice in the handle.
+// residency cap of each row-kernel finalist and the pairs of the capped finalists, and keeps the
+// fastest choice in the handle.
 constexpr int SPMM_FINALISTS = 3;
 constexpr int SPMM_PAIR_ITS = 4;   // even: the iterate ends in the buffer it started in
 extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
@@ -952,8 +1005,8 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                 if (gs == 0 || groups_apply(gs, k, M.n))
                     for (int c = 0; c < (geo ? SPMM_ORDERS : 1); ++c) codes.push_back(c + SPMM_GROUP_CODE * gs);
             for (int ci = 0; ci < STILE_CHUNKS; ++ci)
-                for (int c = 0; c < STILE_SHAPES; ++c)
-                    if (geo && !rays && k % 4 == 0 && k <= 4 * WAVE) codes.push_back(STILE_CODE0 + 10 * ci + c);
+                for (int c = 0; c < (rays ? SRAY_SHAPES : STILE_SHAPES); ++c)
+                    if (geo && k % 4 == 0 && k <= 4 * WAVE) codes.push_back(STILE_CODE0 + 10 * ci + c);
             for (int code : codes) {
                 if (is_tiled(code) && !stiles_apply(h, M, rays, code, k)) continue;
                 auto run = [&] { spmm_launch(h, M, rays, src, k, code, EpiStoreB{Y.p, k}, stream()); };
@@ -966,8 +1019,9 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                 float ms = 0.f;
                 FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
                 if (dbg && is_tiled(code)) {
-                    fprintf(stderr, "[fair] spmm %s tiled %d (%dx%d pixels, LDS-staged union, %d-row chunks): %.1f us\n",
-                            dir == 0 ? "A" : "A^T", code, STILE[tiled_shape(code)][0], STILE[tiled_shape(code)][1],
+                    const int *shp = rays ? SRAY[tiled_shape(code)] : STILE[tiled_shape(code)];
+                    fprintf(stderr, "[fair] spmm %s tiled %d (%dx%d %s, LDS-staged union, %d-row chunks): %.1f us\n",
+                            dir == 0 ? "A" : "A^T", code, shp[0], shp[1], rays ? "rays" : "pixels",
                             tiled_chunk(code), ms * 500.f);
                 } else if (dbg) {
                     const int oc = code_order(code), gs = code_grouping(code);
@@ -984,59 +1038,70 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
         // The finalists' pairs run as whole SART iterations (residual and update epilogues, both
         // matrices and four panels streaming through the caches as in a run), which the single
         // products above do not show: a product read in several passes re-reads its matrix from
-        // HBM there.  The fastest pair is kept.
+        // HBM there.  Then each row-kernel finalist's residency cap is tuned against the other
+        // product's best code, and the pairs of the capped finalists are timed; the fastest is kept.
         DBuf<float> Bz((size_t)A.m * k);
         FAIR_HIP(hipMemsetAsync(Bz.p, 0, Bz.bytes(), stream()));
+        auto iteration_ms = [&](int ca, int ct) {
+            h->spmm_tile[0] = ca;
+            h->spmm_tile[1] = ct;
+            spmm_prepare(h, k);
+            batch_iterations(h, FAIR_SART, Bz.p, k, 2, X.p, R.p, Y.p, nullptr, stream());
+            FAIR_HIP(hipEventRecord(e0, stream()));
+            batch_iterations(h, FAIR_SART, Bz.p, k, SPMM_PAIR_ITS, X.p, R.p, Y.p, nullptr, stream());
+            FAIR_HIP(hipEventRecord(e1, stream()));
+            FAIR_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (dbg) fprintf(stderr, "[fair] spmm orders %d, %d: %.1f us per iteration\n", ca, ct, ms * 1000.f / SPMM_PAIR_ITS);
+            return ms;
+        };
+        // uncapped pairs
         float best = 1e30f;
         int best_pair[2] = {timed[0].empty() ? 0 : timed[0][0].second, timed[1].empty() ? 0 : timed[1][0].second};
         for (const auto &ta : timed[0])
             for (const auto &tt : timed[1]) {
-                h->spmm_tile[0] = ta.second;
-                h->spmm_tile[1] = tt.second;
-                spmm_prepare(h, k);
-                batch_iterations(h, FAIR_SART, Bz.p, k, 2, X.p, R.p, Y.p, nullptr, stream());
-                FAIR_HIP(hipEventRecord(e0, stream()));
-                batch_iterations(h, FAIR_SART, Bz.p, k, SPMM_PAIR_ITS, X.p, R.p, Y.p, nullptr, stream());
-                FAIR_HIP(hipEventRecord(e1, stream()));
-                FAIR_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (dbg)
-                    fprintf(stderr, "[fair] spmm orders %d, %d: %.1f us per iteration\n", ta.second, tt.second,
-                            ms * 1000.f / SPMM_PAIR_ITS);
+                const float ms = iteration_ms(ta.second, tt.second);
                 if (ms < best) {
                     best = ms;
                     best_pair[0] = ta.second;
                     best_pair[1] = tt.second;
                 }
             }
-        // Then each product's residency cap, one product after the other, on whole iterations.
+        // each row-kernel finalist's best cap, against the other product's best code
+        std::vector<int> capped[2];
+        for (int dir = 0; dir < 2; ++dir)
+            for (const auto &tc : timed[dir]) {
+                int bc = tc.second;
+                if (!is_tiled(bc)) {
+                    int trial[2] = {best_pair[0], best_pair[1]};
+                    trial[dir] = tc.second;
+                    float bms = iteration_ms(trial[0], trial[1]);
+                    for (int cap : SPMM_CAPS) {
+                        trial[dir] = tc.second + SPMM_CAP_CODE * cap;
+                        const float ms = iteration_ms(trial[0], trial[1]);
+                        if (ms < bms) {
+                            bms = ms;
+                            bc = trial[dir];
+                        }
+                    }
+                }
+                capped[dir].push_back(bc);
+            }
+        // and the pairs of the capped finalists
+        for (int ca : capped[0])
+            for (int ct : capped[1]) {
+                if (!(ca >= SPMM_CAP_CODE || ct >= SPMM_CAP_CODE)) continue;   // timed uncapped above
+                const float ms = iteration_ms(ca, ct);
+                if (ms < best) {
+                    best = ms;
+                    best_pair[0] = ca;
+                    best_pair[1] = ct;
+                }
+            }
         h->spmm_tile[0] = best_pair[0];
         h->spmm_tile[1] = best_pair[1];
         spmm_prepare(h, k);
-        for (int dir = 0; dir < 2; ++dir) {
-            const int base = h->spmm_tile[dir];
-            if (is_tiled(base)) continue;
-            int best_code = base;
-            for (int cap : SPMM_CAPS) {
-                h->spmm_tile[dir] = base + SPMM_CAP_CODE * cap;
-                batch_iterations(h, FAIR_SART, Bz.p, k, 2, X.p, R.p, Y.p, nullptr, stream());
-                FAIR_HIP(hipEventRecord(e0, stream()));
-                batch_iterations(h, FAIR_SART, Bz.p, k, SPMM_PAIR_ITS, X.p, R.p, Y.p, nullptr, stream());
-                FAIR_HIP(hipEventRecord(e1, stream()));
-                FAIR_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (dbg)
-                    fprintf(stderr, "[fair] spmm orders %d, %d: %.1f us per iteration\n", h->spmm_tile[0],
-                            h->spmm_tile[1], ms * 1000.f / SPMM_PAIR_ITS);
-                if (ms < best) {
-                    best = ms;
-                    best_code = h->spmm_tile[dir];
-                }
-            }
-            h->spmm_tile[dir] = best_code;
-        }
         FAIR_HIP(hipStreamSynchronize(stream()));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

@@ -165,12 +165,14 @@ int fair_sirt_f32_dev(fair_csr *A, int32_t method, const float *d_b, int32_t max
 int fair_sirt_batch_dev(fair_csr *A, int32_t method, const float *d_B, int32_t k, int32_t maxits,
                         float *d_X, const fair_opts *opts, int32_t use_graph);
 /* Autotuner of the batched SpMM: times the row orders (rays in sinogram tiles for A*X, pixels in
- * image tiles for A^T*R; codes 0-11, 0 = stored order) and, for A^T*R, the LDS-staged tiled kernel
- * (codes 100-104: 4x4, 8x2, 2x8, 16x1, 1x16 pixel tiles) on this matrix with k right-hand sides,
- * the row orders also with two rows per wavefront in passes of 128 right-hand sides (code + 1000) and
- * four rows in passes of 64 (code + 2000), then at most b = 2..6 workgroups resident per compute unit
- * for the row kernels (code + 10000 b), and keeps the fastest in the handle (order_out[2] receives the
- * codes, may be NULL).  All choices give the same results. */
+ * image tiles for A^T*R; codes 0-11, 0 = stored order), also with two rows per wavefront in passes of
+ * 128 right-hand sides (code + 1000) and four rows in passes of 64 (code + 2000), and the LDS-staged
+ * tiled kernel (codes 100 + 10 c + i: chunks of 64 (c = 0) or 32 (c = 1) staged rows; pixel tiles
+ * i = 4x4, 8x2, 2x8, 16x1, 1x16, 8x4, 4x8, 8x8, 16x4 for A^T*R, ray tiles i = 8x8, 16x4, 4x16, 8x4,
+ * 4x8, 4x4 angles x detectors for A*X) on this matrix with k right-hand sides; pairs the fastest as
+ * whole SART iterations, then tries at most b = 2..6 workgroups resident per compute unit for the row
+ * kernels (code + 10000 b), and keeps the fastest in the handle (order_out[2] receives the codes, may
+ * be NULL).  All choices give the same results. */
 int fair_autotune_batch(fair_csr *A, int32_t k, int32_t *order_out);
 /* Host-array form of fair_sirt_batch_dev (B: m x k, X0 may be NULL, Xout: n x k). */
 int fair_sirt_batch(fair_csr *A, int32_t method, const float *B, int32_t k, int32_t maxits,
