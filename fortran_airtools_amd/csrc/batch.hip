@@ -456,7 +456,6 @@ __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ bi
                                                      const float *__restrict__ X, int32_t k, Epi epi) {
     extern __shared__ float4 stage[];   // two buffers of CH rows of k / 4 float4
     constexpr int PER_WAVE = CH / STILE_ROWS;
-    constexpr int PAST = 1 << 20;       // slot of an entry past the row's end
     const int b = blockIdx.x;
     const int lb = (b % NXCD) * per_xcd + b / NXCD;
     if (lb >= nblocks) return;          // the whole block leaves together
@@ -486,21 +485,21 @@ __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ bi
     for (int i = 0; i < PER_WAVE; ++i)
         if (w + STILE_ROWS * i < min(CH, bi.w) && lane < kv)
             stage[(w + STILE_ROWS * i) * kv + lane] = make_float4(pre[i].v[0], pre[i].v[1], pre[i].v[2], pre[i].v[3]);
-    // Each row's entries from q on, one per lane.  A chunk holds at most CH <= 64 of a row's entries
-    // (the layout is built only where every row's slots strictly ascend), so one window per chunk
-    // holds them all; the window for chunk c + 1 is loaded at the end of step c, after the gathers
-    // of chunk c + 1 were issued and before they are waited for.  Every step issues the same loads
-    // in the same order (the last step's gathers re-read the union's last row, a missing row loads
-    // entry 0; neither is used), so that the waits between them are exact.
-    int q[RPW], sl[RPW];
+    // Each row's entries from q0 on, one per lane; q is the next one to sum.  A chunk holds at most
+    // CH <= 64 of a row's entries (the layout is built only where every row's slots strictly
+    // ascend), so a window loaded at q holds all entries of the next 64 / CH chunks; the windows are
+    // reloaded that often, all rows together, at the end of a step, after the gathers of the next
+    // chunk were waited for: the waits for a window never wait for gathers issued after it.  Every
+    // step issues the same gathers (the last step's re-read the union's last row, unused) and a
+    // missing row loads entry 0, so that the waits between them are exact.
+    int q0[RPW], q[RPW], sl[RPW];
     float va[RPW];
     float acc[RPW][4];
 #pragma unroll
     for (int j = 0; j < RPW; ++j) {
-        q[j] = ri[j].y;
-        sl[j] = (int)slot[min(q[j] + lane, last[j])];
-        va[j] = vA[min(q[j] + lane, last[j])];
-        if (q[j] + lane >= ri[j].z) sl[j] = PAST;
+        q0[j] = q[j] = ri[j].y;
+        sl[j] = (int)slot[min(q0[j] + lane, last[j])];
+        va[j] = vA[min(q0[j] + lane, last[j])];
         acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 0.f;
     }
     const int nch = (bi.w + CH - 1) / CH;
@@ -516,8 +515,10 @@ __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ bi
         for (int i = 0; i < PER_WAVE; ++i) pre[i] = FV<4>::load(X + (int64_t)col[i] * k + 4 * kl);
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
-            const int nin = has[j] ? __popcll(__ballot(sl[j] < c0 + cn)) : 0;   // a prefix: slots ascend
-            int t = 0;
+            // entries of the window below the chunk's end (a prefix: the slots ascend; lanes past
+            // the row's end hold a clamped copy and do not count), from q on
+            const int nin = has[j] ? __popcll(__ballot(sl[j] < c0 + cn && q0[j] + lane < ri[j].z)) : 0;
+            int t = q[j] - q0[j];
             for (; t + 4 <= nin; t += 4) {
                 float4 xv[4];
                 float wv[4];
@@ -544,13 +545,7 @@ __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ bi
                 acc[j][2] += wv * xv.z;
                 acc[j][3] += wv * xv.w;
             }
-            q[j] += nin;   // the rest of the row lies in later chunks
-        }
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            sl[j] = (int)slot[min(q[j] + lane, last[j])];
-            va[j] = vA[min(q[j] + lane, last[j])];
-            if (q[j] + lane >= ri[j].z) sl[j] = PAST;
+            q[j] = q0[j] + max(nin, q[j] - q0[j]);   // the rest of the row lies in later chunks
         }
         float4 *nb = stage + ((c + 1) & 1) * CH * kv;
         const int nn = min(CH, bi.w - c0 - CH);
@@ -558,6 +553,14 @@ __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ bi
         for (int i = 0; i < PER_WAVE; ++i)
             if (w + STILE_ROWS * i < nn && lane < kv)
                 nb[(w + STILE_ROWS * i) * kv + lane] = make_float4(pre[i].v[0], pre[i].v[1], pre[i].v[2], pre[i].v[3]);
+        if (2 * CH > WAVE || (c & 1)) {   // a window holds the next WAVE / CH chunks' entries of a row
+#pragma unroll
+            for (int j = 0; j < RPW; ++j) {
+                q0[j] = q[j];
+                sl[j] = (int)slot[min(q0[j] + lane, last[j])];
+                va[j] = vA[min(q0[j] + lane, last[j])];
+            }
+        }
     }
 #pragma unroll
     for (int j = 0; j < RPW; ++j)
