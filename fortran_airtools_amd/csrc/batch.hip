@@ -228,6 +228,24 @@ __device__ __forceinline__ void spmm_span(__amdgpu_buffer_rsrc_t px, uint32_t oj
         for (int e = 0; e < 4; ++e) acc[e] += __uint_as_float(w[u]) * __uint_as_float(t[u][e]);
 }
 
+// entries T0 .. T0 + 7 of a 16-entry half window held by each 16-lane group: LDS row offset and
+// weight broadcast from lane t of the group; the lane reads its float4 of the staged row
+template <int T0, int... U>
+__device__ __forceinline__ void spmm_slice_span(const float4 *buf, int off, uint32_t wbits, int gl, float (&acc)[4],
+                                                std::integer_sequence<int, U...>) {
+    const int o[sizeof...(U)] = {(int)row_bcast<T0 + U>((uint32_t)off)...};
+    const uint32_t wv[sizeof...(U)] = {row_bcast<T0 + U>(wbits)...};
+#pragma unroll
+    for (int u = 0; u < (int)sizeof...(U); ++u) {
+        const float4 xv = buf[o[u] + gl];
+        const float ww = __uint_as_float(wv[u]);
+        acc[0] += ww * xv.x;
+        acc[1] += ww * xv.y;
+        acc[2] += ww * xv.z;
+        acc[3] += ww * xv.w;
+    }
+}
+
 template <int G, class Epi>
 __global__ void __launch_bounds__(256) k_spmm_groups(const int4 *__restrict__ rows, int32_t nrows, int32_t per_xcd,
                                                      const int32_t *__restrict__ JA, const float *__restrict__ vA,
@@ -438,16 +456,22 @@ constexpr int STILE_SHAPES = 9;   // 16, 32 or 64 pixels (x by y): one, two or f
 constexpr int STILE[STILE_SHAPES][2] = {{4, 4}, {8, 2}, {2, 8}, {16, 1}, {1, 16}, {8, 4}, {4, 8}, {8, 8}, {16, 4}};
 constexpr int SRAY_SHAPES = 6;    // 16, 32 or 64 rays (angles by detectors)
 constexpr int SRAY[SRAY_SHAPES][2] = {{8, 8}, {16, 4}, {4, 16}, {8, 4}, {4, 8}, {4, 4}};
-constexpr int STILE_CHUNKS = 2;
-constexpr int STILE_CH[STILE_CHUNKS] = {64, 32};
-// order codes STILE_CODE0 + 10 * c + i select the tiled kernel, shape i, chunk STILE_CH[c]
+constexpr int STILE_CHUNKS = 4;
+constexpr int STILE_CH[STILE_CHUNKS] = {64, 32, 128, 64};
+constexpr int STILE_SLICE = 64;   // right-hand sides per pass of the sliced kernel (chunk kinds 2, 3)
+// order codes STILE_CODE0 + 10 * c + i select the tiled kernel, shape i, chunk STILE_CH[c]; c = 0, 1
+// stage whole panel rows (k_spmm_tiles), c = 2, 3 slices of 64 right-hand sides (k_spmm_tiles_sliced)
 constexpr int STILE_CODE0 = 100;
 bool is_tiled(int code) {
     return code >= STILE_CODE0 && code < STILE_CODE0 + 10 * STILE_CHUNKS && code % 10 < STILE_SHAPES;
 }
 int tiled_shape(int code) { return (code - STILE_CODE0) % 10; }
 int tiled_chunk(int code) { return STILE_CH[(code - STILE_CODE0) / 10]; }
-size_t tiled_lds(int code, int32_t k) { return 2 * (size_t)tiled_chunk(code) * (size_t)k * sizeof(float); }
+bool tiled_sliced(int code) { return (code - STILE_CODE0) / 10 >= 2; }
+size_t tiled_lds(int code, int32_t k) {
+    return tiled_sliced(code) ? 2 * ((size_t)tiled_chunk(code) + 1) * STILE_SLICE * sizeof(float)
+                              : 2 * (size_t)tiled_chunk(code) * (size_t)k * sizeof(float);
+}
 
 template <int CH, int RPW, class Epi>
 __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ binfo, int32_t nblocks, int32_t per_xcd,
@@ -565,6 +589,99 @@ __global__ void __launch_bounds__(1024) k_spmm_tiles(const int4 *__restrict__ bi
 #pragma unroll
     for (int j = 0; j < RPW; ++j)
         if (has[j] && lane < kv) epi.template apply<4>(ri[j].x, lane * 4, acc[j]);
+}
+
+// The sliced tiled kernel: the same tiles and layout, the right-hand sides in passes of 64 (the
+// grid's y index), so a staged union row is 256 B and a chunk of CH = 64 or 128 of them holds four
+// times the entries of a row-staged chunk per barrier.  Each 16-lane group of a wavefront takes one
+// row of the tile (64 rows per workgroup), each lane four right-hand sides; a group holds a 32-entry
+// window of its row's entries from the next one to sum (slot and value, two per lane), reloaded per
+// chunk, and hands entry t to its lanes with the DPP row broadcast.  Window positions outside the
+// chunk or the row read the buffer's zero row with weight 0, which adds +0: each output sums its row's entries in the stored order with the same
+// expression as the row kernels, so all give the same bits.
+template <int CH, class Epi>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) k_spmm_tiles_sliced(const int4 *__restrict__ binfo, int32_t nblocks,
+                                                            int32_t per_xcd, const int4 *__restrict__ rows,
+                                                            const int32_t *__restrict__ uni,
+                                                            const uint16_t *__restrict__ slot,
+                                                            const float *__restrict__ vA, const float *__restrict__ X,
+                                                            int32_t k, Epi epi) {
+    extern __shared__ float4 stage[];   // two buffers of CH + 1 rows of 16 float4; row CH stays zero
+    constexpr int RW = STILE_SLICE / 4;   // float4 per staged row
+    constexpr int PT = CH * RW / 1024;    // staged float4 per thread and chunk
+    static_assert(PT >= 1 && CH * RW == PT * 1024, "a chunk is a whole number of float4 per thread");
+    const int b = blockIdx.x;
+    const int lb = (b % NXCD) * per_xcd + b / NXCD;
+    if (lb >= nblocks) return;          // the whole block leaves together
+    const int4 bi = binfo[lb];          // first row, rows, union start, union size
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & (WAVE - 1)), gl = lane & 15;
+    const int ri_i = 4 * w + (lane >> 4);   // this lane group's row of the tile
+    const bool has = ri_i < bi.y;
+    const int4 ri = has ? rows[bi.x + ri_i] : make_int4(0, 0, 0, 0);
+    const int last = max(ri.z - 1, ri.y);
+    const int c0s = (int)blockIdx.y * STILE_SLICE;
+    const int col = c0s + 4 * gl;           // this lane's right-hand sides
+    // staging: thread t copies float4 (t % 16) of rows t / 16 + 64 i; rows past the union re-read its
+    // last one (not stored)
+    const int sr = (int)threadIdx.x >> 4, sc = (int)threadIdx.x & 15;
+    const int scol = min(c0s + 4 * sc, k - 4);
+    FV<4> pre[PT];
+    int ucol[PT];
+#pragma unroll
+    for (int i = 0; i < PT; ++i) ucol[i] = uni[bi.z + min(sr + 64 * i, bi.w - 1)];
+#pragma unroll
+    for (int i = 0; i < PT; ++i) pre[i] = FV<4>::load(X + (int64_t)ucol[i] * k + scol);
+#pragma unroll
+    for (int i = 0; i < PT; ++i)
+        if (sr + 64 * i < min(CH, bi.w))
+            stage[(sr + 64 * i) * RW + sc] = make_float4(pre[i].v[0], pre[i].v[1], pre[i].v[2], pre[i].v[3]);
+    if (threadIdx.x < 2 * RW) stage[((threadIdx.x / RW) * (CH + 1) + CH) * RW + threadIdx.x % RW] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // the row's 32-entry window from w0 (its next entry to sum): entries w0 + gl and w0 + 16 + gl
+    int w0 = ri.y;
+    int sl0 = (int)slot[min(w0 + gl, last)], sl1 = (int)slot[min(w0 + 16 + gl, last)];
+    float va0 = vA[min(w0 + gl, last)], va1 = vA[min(w0 + 16 + gl, last)];
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    const int nch = (bi.w + CH - 1) / CH;
+    for (int c = 0; c < nch; ++c) {
+        const int c0 = c * CH, cn = min(CH, bi.w - c0);
+        const float4 *buf = stage + (c & 1) * (CH + 1) * RW;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PT; ++i) ucol[i] = uni[bi.z + min(c0 + CH + sr + 64 * i, bi.w - 1)];
+#pragma unroll
+        for (int i = 0; i < PT; ++i) pre[i] = FV<4>::load(X + (int64_t)ucol[i] * k + scol);
+        for (;;) {
+            // the window starts at q (w0 == q): its positions in this chunk are a prefix (the slots
+            // ascend), of at most 32 entries per group; 8-entry spans no group needs are skipped
+            const int e0 = w0 + gl, e1 = w0 + 16 + gl;
+            const bool in0 = has && e0 < ri.z && sl0 < c0 + cn;
+            const bool in1 = has && e1 < ri.z && sl1 < c0 + cn;
+            const int o0 = (in0 ? sl0 - c0 : CH) * RW, o1 = (in1 ? sl1 - c0 : CH) * RW;
+            const uint32_t u0 = in0 ? __float_as_uint(va0) : 0u, u1 = in1 ? __float_as_uint(va1) : 0u;
+            const uint64_t m0 = __ballot(in0), m1 = __ballot(in1);
+            constexpr uint64_t LO = 0x00ff00ff00ff00ffull, HI = 0xff00ff00ff00ff00ull;
+            if (m0 & LO) spmm_slice_span<0>(buf, o0, u0, gl, acc, std::make_integer_sequence<int, 8>{});
+            if (m0 & HI) spmm_slice_span<8>(buf, o0, u0, gl, acc, std::make_integer_sequence<int, 8>{});
+            if (m1 & LO) spmm_slice_span<0>(buf, o1, u1, gl, acc, std::make_integer_sequence<int, 8>{});
+            if (m1 & HI) spmm_slice_span<8>(buf, o1, u1, gl, acc, std::make_integer_sequence<int, 8>{});
+            const int sh = lane & ~15;
+            const int cnt = __popc((uint32_t)(m0 >> sh) & 0xffffu) + __popc((uint32_t)(m1 >> sh) & 0xffffu);
+            w0 += cnt;   // the next window starts at the next entry
+            const bool more = has && cnt == 32 && w0 < ri.z;   // the window lay wholly in the chunk
+            sl0 = (int)slot[min(w0 + gl, last)];
+            sl1 = (int)slot[min(w0 + 16 + gl, last)];
+            va0 = vA[min(w0 + gl, last)];
+            va1 = vA[min(w0 + 16 + gl, last)];
+            if (!__ballot(more)) break;
+        }
+        float4 *nb = stage + ((c + 1) & 1) * (CH + 1) * RW;
+        const int nn = min(CH, bi.w - c0 - CH);
+#pragma unroll
+        for (int i = 0; i < PT; ++i)
+            if (sr + 64 * i < nn) nb[(sr + 64 * i) * RW + sc] = make_float4(pre[i].v[0], pre[i].v[1], pre[i].v[2], pre[i].v[3]);
+    }
+    if (has && col < k) epi.template apply<4>(ri.x, col, acc);
 }
 
 // The tiled layout of M for tile shape i: rows are pixels (j = x * N + y, tiles of STILE[i] pixels)
@@ -686,7 +803,7 @@ bool build_stiles(const fair_csr *h, Csr &M, bool rays, int shape) {
 // Whether order code `code` selects the tiled kernel for M with k right-hand sides (and its layout
 // is built).
 bool stiles_apply(const fair_csr *h, Csr &M, bool rays, int code, int32_t k) {
-    if (!is_tiled(code) || k % 4 != 0 || k > 4 * WAVE) return false;
+    if (!is_tiled(code) || k % 4 != 0 || (!tiled_sliced(code) && k > 4 * WAVE)) return false;
     if (tiled_lds(code, k) > (size_t)lds_per_block()) return false;
     return build_stiles(h, M, rays, tiled_shape(code));
 }
@@ -697,6 +814,13 @@ void spmm_launch_tiles(const Csr &M, bool rays, const float *X, int32_t k, int c
     const int64_t per_xcd = (blocks + NXCD - 1) / NXCD;
     const size_t lds = tiled_lds(code, k);
     const int *shp = rays ? SRAY[tiled_shape(code)] : STILE[tiled_shape(code)];
+    if (tiled_sliced(code)) {
+        auto kern = tiled_chunk(code) == 128 ? k_spmm_tiles_sliced<128, Epi> : k_spmm_tiles_sliced<64, Epi>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)(per_xcd * NXCD), (unsigned)((k + STILE_SLICE - 1) / STILE_SLICE)),
+                           dim3(1024), lds, s, (const int4 *)M.stile.binfo.p, (int32_t)blocks, (int32_t)per_xcd,
+                           (const int4 *)M.stile.rows.p, M.stile.uni.p, M.stile.slot.p, M.vA32.p, X, k, epi);
+        return;
+    }
     const int rpw = shp[0] * shp[1] / STILE_ROWS;
     auto kern = tiled_chunk(code) == 64
                     ? (rpw == 4 ? k_spmm_tiles<64, 4, Epi> : rpw == 2 ? k_spmm_tiles<64, 2, Epi> : k_spmm_tiles<64, 1, Epi>)
@@ -1009,7 +1133,7 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                     for (int c = 0; c < (geo ? SPMM_ORDERS : 1); ++c) codes.push_back(c + SPMM_GROUP_CODE * gs);
             for (int ci = 0; ci < STILE_CHUNKS; ++ci)
                 for (int c = 0; c < (rays ? SRAY_SHAPES : STILE_SHAPES); ++c)
-                    if (geo && k % 4 == 0 && k <= 4 * WAVE) codes.push_back(STILE_CODE0 + 10 * ci + c);
+                    if (geo && k % 4 == 0 && (ci >= 2 || k <= 4 * WAVE)) codes.push_back(STILE_CODE0 + 10 * ci + c);
             for (int code : codes) {
                 if (is_tiled(code) && !stiles_apply(h, M, rays, code, k)) continue;
                 auto run = [&] { spmm_launch(h, M, rays, src, k, code, EpiStoreB{Y.p, k}, stream()); };
@@ -1023,9 +1147,9 @@ extern "C" int fair_autotune_batch(fair_csr *h, int32_t k, int32_t *order_out) {
                 FAIR_HIP(hipEventElapsedTime(&ms, e0, e1));
                 if (dbg && is_tiled(code)) {
                     const int *shp = rays ? SRAY[tiled_shape(code)] : STILE[tiled_shape(code)];
-                    fprintf(stderr, "[fair] spmm %s tiled %d (%dx%d %s, LDS-staged union, %d-row chunks): %.1f us\n",
+                    fprintf(stderr, "[fair] spmm %s tiled %d (%dx%d %s, LDS-staged union, %d-row chunks%s): %.1f us\n",
                             dir == 0 ? "A" : "A^T", code, shp[0], shp[1], rays ? "rays" : "pixels",
-                            tiled_chunk(code), ms * 500.f);
+                            tiled_chunk(code), tiled_sliced(code) ? " of 64 right-hand sides" : "", ms * 500.f);
                 } else if (dbg) {
                     const int oc = code_order(code), gs = code_grouping(code);
                     const int *tl = rays ? RAY_TILES[oc] : PIX_TILES[oc];

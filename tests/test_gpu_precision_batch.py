@@ -98,11 +98,13 @@ def test_batch_graph_survives_workspace_and_matrix_changes():
 
 
 @pytest.mark.parametrize("geom", ["fan_c1", "parallel_n96_pi"])
-@pytest.mark.parametrize("tile_code", [100, 101, 102, 103, 104, 105, 110, 111, 112, 113, 114, 115])
+@pytest.mark.parametrize("tile_code", [100, 101, 102, 103, 104, 105, 110, 111, 112, 113, 114, 115,
+                                       120, 121, 123, 125, 130, 133, 135])
 def test_tiled_ray_product_equals_row_kernel(monkeypatch, geom, tile_code):
     """The LDS-staged tiled kernel for A*X (rays of a sinogram tile of 8 x 8 to 4 x 4 angles x
     detectors, their pixels' union staged in an order every ray's stored order follows; tiles
-    without such an order are split) gives the same bits as the wavefront-per-row kernel.  The fan
+    without such an order are split; codes 12x / 13x stage slices of 64 right-hand sides) gives the
+    same bits as the wavefront-per-row kernel.  The fan
     matrix has tiles that need splitting; the parallel-beam one has columns outside the image, for
     which the layout does not apply and the row kernel runs.  k = 32 and 256."""
     import torch
@@ -151,12 +153,14 @@ def test_batch_graph_after_row_order_changes(monkeypatch):
 
 
 @pytest.mark.parametrize("tile_code", [100, 101, 102, 103, 104, 105, 106, 107, 108,
-                                       110, 111, 112, 113, 114, 115, 116, 117, 118])
+                                       110, 111, 112, 113, 114, 115, 116, 117, 118,
+                                       120, 125, 127, 128, 130, 135, 137])
 def test_tiled_transpose_product_equals_row_kernel(monkeypatch, tile_code):
     """The LDS-staged tiled kernel of A^T*R (pixels of an image tile, their rays' union staged in
     double-buffered chunks of 64 (codes 100-108) or 32 (110-118) rows; 16-pixel tiles one pixel per
-    wavefront, 8 x 4 and 4 x 8 tiles (codes x05, x06) two, 8 x 8 and 16 x 4 (x07, x08) four) gives
-    the same bits as the
+    wavefront, 8 x 4 and 4 x 8 tiles (codes x05, x06) two, 8 x 8 and 16 x 4 (x07, x08) four; codes
+    12x and 13x stage slices of 64 right-hand sides in chunks of 128 and 64 rows, one pixel per
+    16-lane group; k = 32 leaves half of each group idle) gives the same bits as the
     wavefront-per-row kernel: every pixel's sum runs over its entries in the stored order with the
     same operations.  k = 32 and k = 256 right-hand sides."""
     import torch
