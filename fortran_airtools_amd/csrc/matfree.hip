@@ -112,6 +112,12 @@ __device__ __forceinline__ void ray_line(const MfArgs &g, int i, int j, double &
 // floor(x0 + n/2) (half-open pixels), likewise b == 0.
 struct Walk {
     double x0, y0, h, inva, invb, len, t, t1, tx, ty;
+    // Per axis: dx = L - n/2 for the next grid line L (exact: integers and halves), and the line's
+    // t = (dx - qx) * kx, i.e. tline's ((L - n/2) - x0) * (1/a) with the same roundings.  A step adds
+    // +-1 to dx; the other axis recomputes its unchanged t.  An axis the ray runs parallel to keeps
+    // t = 1e300 (dx = 1e300, qx = 0, kx = 1, no steps), which compares like the infinity it stands for.
+    double dx, dy, qx, qy, kx, ky;
+    int sxhi, syhi;   // high words of the doubles +-1.0 (their low words are 0), or 0 for no steps
     int ix, iy, sx, sy, ixe, iye;
     bool live;
 
@@ -166,8 +172,30 @@ struct Walk {
         if (b == 0.0) iy = (int)floor(y0 + h);
         else if (tye >= txe) iy = b > 0.0 ? by0 : by1 - 1;
         else iy = min(max((int)floor(y0 + t0 * b + h), by0), by1 - 1);
-        tx = a != 0.0 ? tX(ix + (sx > 0)) : INFINITY;
-        ty = b != 0.0 ? tY(iy + (sy > 0)) : INFINITY;
+        if (a != 0.0) {
+            dx = (double)(ix + (sx > 0)) - h;
+            qx = x0;
+            kx = inva;
+            sxhi = __double2hiint((double)sx);
+        } else {
+            dx = 1e300;
+            qx = 0.0;
+            kx = 1.0;
+            sxhi = 0;
+        }
+        if (b != 0.0) {
+            dy = (double)(iy + (sy > 0)) - h;
+            qy = y0;
+            ky = invb;
+            syhi = __double2hiint((double)sy);
+        } else {
+            dy = 1e300;
+            qy = 0.0;
+            ky = 1.0;
+            syhi = 0;
+        }
+        tx = (dx - qx) * kx;   // = tX(ix + (sx > 0)) for a != 0
+        ty = (dy - qy) * ky;
         ixe = sx > 0 ? bx1 : bx0 - 1;
         iye = sy > 0 ? by1 : by0 - 1;
         t = t0;
@@ -188,9 +216,10 @@ struct Walk {
         ix += stepx ? sx : 0;
         iy += stepx ? 0 : sy;
         live = tn < t1 && ix != ixe && iy != iye;
-        const double tnew = tline(stepx ? ix + (sx > 0) : iy + (sy > 0), stepx ? x0 : y0, stepx ? inva : invb);
-        tx = stepx ? tnew : tx;
-        ty = stepx ? ty : tnew;
+        dx += __hiloint2double(stepx ? sxhi : 0, 0);
+        dy += __hiloint2double(stepx ? 0 : syhi, 0);
+        tx = (dx - qx) * kx;
+        ty = (dy - qy) * ky;
     }
 };
 
