@@ -121,7 +121,17 @@ struct Walk {
     double dx, dy, qx, qy, kx, ky;
     int sxhi, syhi;   // high words of the doubles +-1.0 (their low words are 0), or 0 for no steps
     int ix, iy, sx, sy, ixe, iye;
+    // a caller's linear index of pixel (ix, iy), stepped with ix and iy (set by track)
+    int lin, linx, liny;
     bool live;
+
+    // lin = index of the current pixel; it changes by dx_lin when ix steps by sx, by dy_lin when iy
+    // steps by sy
+    __device__ __forceinline__ void track(int lin0, int dx_lin, int dy_lin) {
+        lin = lin0;
+        linx = dx_lin;
+        liny = dy_lin;
+    }
 
     // t of the grid line with index L: ((L - n/2) - x0) * (1/a).  L - n/2 is exact, so a line
     // through the ray's own start coordinate gets t = 0 exactly, as the reference's
@@ -217,6 +227,24 @@ struct Walk {
         const bool stepx = tx <= ty;
         ix += stepx ? sx : 0;
         iy += stepx ? 0 : sy;
+        live = tn < t1 && ix != ixe && iy != iye;
+        dx += __hiloint2double(stepx ? sxhi : 0, 0);
+        dy += __hiloint2double(stepx ? 0 : syhi, 0);
+        tx = (dx - qx) * kx;
+        ty = (dy - qy) * ky;
+    }
+
+    // next() for a tracked walk: the segment's pixel as its linear index
+    __device__ __forceinline__ void next_lin(int &olin, float &seg) {
+        const double tn = fmin(fmin(tx, ty), t1);
+        const double d = (tn - t) * len;
+        seg = d > 1e-10 ? (float)d : 0.f;
+        olin = lin;
+        t = tn;
+        const bool stepx = tx <= ty;
+        ix += stepx ? sx : 0;
+        iy += stepx ? 0 : sy;
+        lin += stepx ? linx : liny;
         live = tn < t1 && ix != ixe && iy != iye;
         dx += __hiloint2double(stepx ? sxhi : 0, 0);
         dy += __hiloint2double(stepx ? 0 : syhi, 0);
@@ -419,11 +447,12 @@ __global__ void __launch_bounds__(256) k_mf_adjoint_tiles(MfArgs g, int32_t nty,
         ray_line(g, i, j, x0, y0, a, bb);
         Walk w;
         w.init(x0, y0, a, bb, n, X0, X1, Y0, Y1);
+        if (w.live) w.track((w.ix - X0) * MF_LDS_STRIDE + (w.iy - Y0), w.sx * MF_LDS_STRIDE, w.sy);
         while (w.live) {
-            int ix, iy;
+            int q;
             float sg;
-            w.next(ix, iy, sg);
-            atomicAdd(&acc[(ix - X0) * MF_LDS_STRIDE + (iy - Y0)], fix_round(rs * (double)sg));
+            w.next_lin(q, sg);
+            atomicAdd(&acc[q], fix_round(rs * (double)sg));
         }
     }
     __syncthreads();
