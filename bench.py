@@ -279,6 +279,8 @@ def main() -> int:
                          "phantoms: every phantom of the gallery and add_white_noise at N=2048 and N=256, "
                          "the reference's own functions timed on the host at N=256")
     ap.add_argument("--uneven-count", type=int, default=64)
+    ap.add_argument("--uneven-sizes", default="", help="comma-separated image sizes for --mode uneven "
+                    "(default: the C5 set, 128..1024)")
     ap.add_argument("--batch-iters", type=int, default=300)
     ap.add_argument("--mf-iters", type=int, default=100)
     ap.add_argument("--calibrate", action="store_true",
@@ -802,13 +804,15 @@ def bench_phantoms(a, np, torch) -> int:
 
 
 def bench_uneven(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
-    """The uneven batch: 64 seeded SART problems (N from 128 to 1024, N/2..2N angles, gallery
-    phantoms, 1 % noise, box [0,1]), each with its own device-built matrix, iterated 20 times as
-    one group (two persistent launches per iteration for all problems) against the same problems
-    run one after another; the group's iterates must equal the single runs bit for bit."""
+    """The uneven batch: 64 seeded SART problems (N from 128 to 1024, or the --uneven-sizes set;
+    N/2..2N angles, gallery phantoms, 1 % noise, box [0,1]), each with its own device-built matrix,
+    iterated 20 times as one group (two persistent launches per iteration for all problems) against
+    the same problems run one after another; the group's iterates must equal the single runs bit
+    for bit."""
     dev = torch.device("cuda:0")
     iters = 20
-    probs = cf.c5_problems(a.uneven_count, 5)
+    sizes = [int(v) for v in a.uneven_sizes.split(",")] if a.uneven_sizes else None
+    probs = cf.c5_problems(a.uneven_count, 5, sizes)
     gens = [cf.shepplogan, cf.bubbles, cf.smooth, cf.threephases]
     o = airms.Opts().set_lbound(0.0).set_ubound(1.0)
     mats, db = [], []
@@ -865,7 +869,8 @@ def bench_uneven(a, np, torch, _lib, airms, cf, sg, L, stream) -> int:
     bytes_it = sum(24 * A.nnz + 4 * (A.m + A.n + 2) + 32 * A.m + 32 * A.n for A in mats)
     out = {"metric": METRIC, "mode": "uneven", "unit": "problem-iterations/s", "dtype": "f64",
            "value": round(len(mats) * iters / t_group, 2),
-           "config": {"workload": f"{len(mats)} fan-beam SART problems, N 128..1024, N/2..2N angles, box [0,1], "
+           "config": {"workload": f"{len(mats)} fan-beam SART problems, N {min(g.n for g in probs)}.."
+                                  f"{max(g.n for g in probs)}, N/2..2N angles, box [0,1], "
                                   f"{iters} iterations, one group", "total_nnz": nnz,
                       "sizes": [g.n for g in probs]},
            "group_s": round(t_group, 5), "sequential_s": round(t_seq, 5), "speedup": round(t_seq / t_group, 3),

@@ -64,12 +64,13 @@ C4 = Geometry(n=2048, p=2900, nth=2048, deg_step=360.0 / 2048)
 C5_SIZES = (128, 192, 256, 384, 512, 768, 1024)
 
 
-def c5_problems(count: int = 64, seed: int = 5) -> list:
-    """The uneven problem set: `count` geometries from one seeded generator."""
+def c5_problems(count: int = 64, seed: int = 5, sizes=None) -> list:
+    """The uneven problem set: `count` geometries from one seeded generator (image sizes drawn
+    from `sizes`, C5_SIZES by default)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
-        n = int(rng.choice(C5_SIZES))
+        n = int(rng.choice(C5_SIZES if sizes is None else sizes))
         nth = int(rng.integers(n // 2, 2 * n + 1))
         h = n // 2
         p = h + 1 if h % 2 == 0 else h
