@@ -2,27 +2,6 @@
 //
 // No counterpart in the reference (its to-do "make matrix-free implementations", README.md:7);
 // the geometry is the one of fanlineartomo (system_generation.f90:98-157): the same angle and ray
-// tables, uploaded from the host, so that at small N the operator equals the stored CSR matrix
-// up to FP32 rounding of the entries.
-//
-// One thread per ray walks the pixels it crosses (Siddon / incremental traversal): the ray is
-// clipped to [-n/2, n/2]^2, then every step moves to the nearer of the next x-line and the next
-// y-line crossing.  Crossing parameters are computed directly from the line index in FP64
-// (t = (line - x0) / a), never accumulated, so long rays do not drift; the pixel values, the
-// segment lengths used in the products and the accumulation are FP32.  The adjoint walks the
-// same segments and scatters r_i * length into the image with FP32 atomic adds, so it is the
-// exact transpose of the forward walk (the dot-product test checks it).
-#include <algorithm>
-#include <cmath>
-#include <memory>
-#include <vector>
-
-#include "fair_internal.hpp"
-
-// matfree.hip -- matrix-free fan-beam projector: A*x and its adjoint recomputed per call.
-//
-// No counterpart in the reference (its to-do "make matrix-free implementations", README.md:7);
-// the geometry is the one of fanlineartomo (system_generation.f90:98-157): the same angle and ray
 // tables, computed on the host, so that at small N the operator equals the stored CSR matrix up
 // to FP32 rounding of the entries.
 //
@@ -34,8 +13,9 @@
 // lengths and sums are FP32.
 //
 //  forward  one thread per ray over the whole image; MF_BATCH segments are generated before
-//           their gathers are issued, so that many loads of x are in flight per thread (the
-//           gathers hit the last-level cache, and latency bounds a one-load-at-a-time walk).
+//           their gathers are issued, and the next MF_BATCH are walked while those are in
+//           flight (the gathers hit the last-level cache, and latency bounds a one-load-at-a-time
+//           walk).
 //  adjoint  one workgroup per (TS x TS image tile, block of MF_AB angles): for each angle the
 //           rays that can cross the tile form a contiguous range of each monotone run of
 //           tan(omega) (found by binary search); their segments inside the tile are accumulated
@@ -45,6 +25,8 @@
 //           the sums (the dot-product test checks it), and it is deterministic.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -74,15 +56,16 @@ struct fair_mf {
     int32_t n_exact = 0;
     int64_t ex_segments = 0;    // table entries of positive length
     fair::DBuf<uint8_t> flag;   // per ray: 1 = in the table (skipped by the traversal kernels)
-    // forward workspace: x transposed (see k_mf_transpose); calls on one handle must not overlap
-    fair::DBuf<float> xT;
+    // forward workspace: a copy of x, then x transposed (see k_mf_transpose); calls on one
+    // handle must not overlap
+    fair::DBuf<float> xx;
+    bool mf_gather_buffer = true;   // gathers through a buffer resource (workspace below 4 GiB)
 };
 
 namespace fair {
 namespace {
 
-// segments per gather batch (forward); on a 2048^2 image with 2048 x 2900 rays 8 took 17.3 ms per
-// product, 12 16.4 ms, 16 17.0 ms (86 VGPRs)
+// segments per gather batch (forward)
 constexpr int MF_BATCH = 12;
 constexpr int MF_TS = 64;     // adjoint image tile side
 constexpr int MF_AB = 64;     // adjoint angles per workgroup
@@ -253,30 +236,45 @@ struct Walk {
     }
 };
 
-// xT[ry * n + ix] = x[ix * n + ry] (ry = n - 1 - iy counts down y, as the columns of A do).
-__global__ void __launch_bounds__(256) k_mf_transpose(const float *__restrict__ x, float *__restrict__ xT, int n) {
+// The forward's workspace xx: xx[i] = x[i] for i < n^2, then xT[ry * n + ix] = x[ix * n + ry]
+// (ry = n - 1 - iy counts down y, as the columns of A do) at xx + n^2, so that one base address
+// and a 32-bit element offset reach either copy.
+__global__ void __launch_bounds__(256) k_mf_transpose(const float *__restrict__ x, float *__restrict__ xx, int n) {
     __shared__ float t[32][33];
     const int rb = blockIdx.x * 32, ib = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int64_t nn = (int64_t)n * n;
     for (int k = ty; k < 32; k += 8) {
         const int ix = ib + k, ry = rb + tx;
-        if (ix < n && ry < n) t[k][tx] = x[(int64_t)ix * n + ry];
+        if (ix < n && ry < n) {
+            const float v = x[(int64_t)ix * n + ry];
+            t[k][tx] = v;
+            xx[(int64_t)ix * n + ry] = v;
+        }
     }
     __syncthreads();
     for (int k = ty; k < 32; k += 8) {
         const int ry = rb + k, ix = ib + tx;
-        if (ix < n && ry < n) xT[(int64_t)ry * n + ix] = t[tx][k];
+        if (ix < n && ry < n) xx[nn + (int64_t)ry * n + ix] = t[tx][k];
     }
 }
 
 // One thread per ray.  Neighbouring lanes hold neighbouring rays of one angle, which at a given
 // step sit one pixel apart across the ray: for rays running mostly along x that is the next
 // column index of x, for rays running mostly along y the next row, n entries away.  Those lanes
-// gather from xT instead, so that a gather instruction touches a few cache lines rather than one
-// per lane.  The values and the order of the sums are the same either way.
-__global__ void __launch_bounds__(256) k_mf_forward(MfArgs g, int64_t m, const float *__restrict__ x,
-                                                    const float *__restrict__ xT, const float *__restrict__ b,
-                                                    float *__restrict__ y) {
+// gather from the transposed copy instead, so that a gather instruction touches a few cache lines
+// rather than one per lane.  The values and the order of the sums are the same either way.
+//
+// The walk is software-pipelined: the gathers of one batch of MF_BATCH segments are issued, the
+// next batch is walked while they are in flight, and only then are they summed.  A gather is a
+// 32-bit element offset into the workspace (BUF: through a buffer resource, no 64-bit address
+// registers; otherwise added to the base pointer).  With 64-bit addresses the compiler reuses a
+// few address registers across a batch's loads, and a VALU write to an address register waits
+// until the load before has read it, which serialises the batch (measured: as slow as the walk
+// without pipelining).
+template <bool BUF>
+__global__ void __launch_bounds__(256) k_mf_forward(MfArgs g, int64_t m, const float *__restrict__ xx,
+                                                    const float *__restrict__ b, float *__restrict__ y) {
     const int64_t ray = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ray >= m || g.flag[ray]) return;
     const int i = (int)(ray / g.p), j = (int)(ray - (int64_t)i * g.p);
@@ -284,25 +282,44 @@ __global__ void __launch_bounds__(256) k_mf_forward(MfArgs g, int64_t m, const f
     ray_line(g, i, j, x0, y0, a, bb);
     const int n = g.n;
     const bool ymajor = fabs(bb) > fabs(a);
-    const float *__restrict__ src = ymajor ? xT : x;
+    const uint32_t nn = (uint32_t)n * (uint32_t)n, base = ymajor ? nn : 0u;
+    // BUF: the host checked that 2 n^2 floats fit the resource's 32-bit byte range
+    const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(xx), (short)0,
+                                                                         BUF ? (int)(8u * nn) : 0, 0x00020000);
     Walk w;
     w.init(x0, y0, a, bb, n, 0, n, 0, n);
-    float acc = 0.f;
-    while (w.live) {
-        int pix[MF_BATCH];
-        float sg[MF_BATCH];
+    // gather offsets, in bytes for BUF (the loads take them as they are), in floats otherwise
+    uint32_t off[MF_BATCH];
+    float sg[MF_BATCH];
+    // every lane walks the first batch: off[] = 0 where the walk is not live
+    auto walk_batch = [&]() {
 #pragma unroll
         for (int k = 0; k < MF_BATCH; ++k) {
-            pix[k] = 0;
+            off[k] = 0;
             sg[k] = 0.f;
             if (w.live) {
                 int ix, iy;
                 w.next(ix, iy, sg[k]);
-                pix[k] = ymajor ? (n - 1 - iy) * n + ix : ix * n + (n - 1 - iy);
+                const uint32_t e = base + (uint32_t)(ymajor ? (n - 1 - iy) * n + ix : ix * n + (n - 1 - iy));
+                off[k] = BUF ? 4u * e : e;
             }
         }
+    };
+    walk_batch();
+    float acc = 0.f;
+    bool more = true;
+    while (more) {
+        float v[MF_BATCH], sc[MF_BATCH];
 #pragma unroll
-        for (int k = 0; k < MF_BATCH; ++k) acc = fmaf(sg[k], src[pix[k]], acc);
+        for (int k = 0; k < MF_BATCH; ++k) {
+            if (BUF) v[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, off[k], 0, 0));
+            else v[k] = xx[off[k]];
+            sc[k] = sg[k];
+        }
+        more = w.live;
+        if (more) walk_batch();
+#pragma unroll
+        for (int k = 0; k < MF_BATCH; ++k) acc = fmaf(sc[k], v[k], acc);
     }
     y[ray] = b ? acc - b[ray] : acc;
 }
@@ -551,9 +568,13 @@ MfArgs args(const fair_mf *h) {
 void mf_forward(fair_mf *h, const float *x, const float *b, float *y) {
     if (h->m > 0) {
         const unsigned nb = (unsigned)((h->n + 31) / 32);
-        hipLaunchKernelGGL(k_mf_transpose, dim3(nb, nb), dim3(256), 0, stream(), x, h->xT.p, h->n);
-        hipLaunchKernelGGL(k_mf_forward, dim3(grid_for(h->m, 256)), dim3(256), 0, stream(), args(h), h->m, x, h->xT.p,
-                           b, y);
+        hipLaunchKernelGGL(k_mf_transpose, dim3(nb, nb), dim3(256), 0, stream(), x, h->xx.p, h->n);
+        if (h->mf_gather_buffer)
+            hipLaunchKernelGGL(k_mf_forward<true>, dim3(grid_for(h->m, 256)), dim3(256), 0, stream(), args(h), h->m,
+                               h->xx.p, b, y);
+        else
+            hipLaunchKernelGGL(k_mf_forward<false>, dim3(grid_for(h->m, 256)), dim3(256), 0, stream(), args(h), h->m,
+                               h->xx.p, b, y);
     }
     if (h->n_exact > 0)
         hipLaunchKernelGGL(k_mf_table_forward, dim3(grid_for(h->n_exact, 64)), dim3(64), 0, stream(), h->ex_rays.p,
@@ -672,7 +693,11 @@ extern "C" int fair_mf_create(const fair_geom *g, const double *theta, fair_mf *
         FAIR_CHECK(h->ecap >= 24, FAIR_ERR_OVERFLOW, "too many rays for the fixed-point adjoint");
         const int64_t nn = (int64_t)n * n;
         h->xacc.alloc(nn);
-        h->xT.alloc(nn);
+        h->xx.alloc(2 * nn);
+        // a buffer resource spans at most 4 GiB - 1 bytes; FAIR_MF_GATHER=pointer selects the
+        // pointer path for any size (the results are the same)
+        const char *gather = getenv("FAIR_MF_GATHER");
+        h->mf_gather_buffer = 8 * nn < ((int64_t)1 << 32) && !(gather && strcmp(gather, "pointer") == 0);
         h->rmax.alloc(1);
         FAIR_HIP(hipMemsetAsync(h->xacc.p, 0, sizeof(unsigned long long) * nn, stream()));
         FAIR_HIP(hipMemsetAsync(h->rmax.p, 0, sizeof(unsigned), stream()));
