@@ -82,11 +82,22 @@ __device__ __forceinline__ V row_direct(int s, int e, int lane, const int32_t *_
                 c[u] = int2v{-1, -1};
             }
         }
+        // Every gather is issued, with an out-of-range column clamped to 0 (a matrix with entries
+        // has ncols >= 1), so that all of them are in flight together; a term outside the row or
+        // the matrix then adds +0, which leaves a sum that starts at +0 unchanged.
+        V g0[SPMV_UNROLL], g1[SPMV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < SPMV_UNROLL; ++u) {
+            g0[u] = x[(uint32_t)c[u].x < (uint32_t)ncols ? c[u].x : 0];
+            g1[u] = x[(uint32_t)c[u].y < (uint32_t)ncols ? c[u].y : 0];
+        }
 #pragma unroll
         for (int u = 0; u < SPMV_UNROLL; ++u) {
             const int k = base + 2 * WAVE * u;
-            if (k >= s && k < e) acc0 += v[u].x * gather(x, c[u].x, ncols);
-            if (k + 1 < e) acc1 += v[u].y * gather(x, c[u].y, ncols);
+            const bool in0 = k >= s && k < e && (uint32_t)c[u].x < (uint32_t)ncols;
+            const bool in1 = k + 1 < e && (uint32_t)c[u].y < (uint32_t)ncols;
+            acc0 += in0 ? v[u].x * g0[u] : (V)0;
+            acc1 += in1 ? v[u].y * g1[u] : (V)0;
         }
     }
     return wave_sum_dpp(acc0 + acc1);
